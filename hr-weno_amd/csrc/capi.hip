@@ -1,0 +1,1000 @@
+// capi.hip -- extern "C" entry points of libhrweno_hip.so (declared in include/hrweno_capi.h).
+//
+// Host side of the MI355X path: objects mirroring the reference's derived types (weno,
+// rktvd/mstvd, grid1), the reference's loop control and error texts, and the dispatch of the
+// gfx950 kernels in kernels_1d.hpp / kernels_2d.hpp.  All arithmetic on cell data runs in
+// those kernels; the host only walks time, counts evaluations and owns buffers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hrweno_capi.h"
+#include "kernels_1d.hpp"
+#include "kernels_2d.hpp"
+
+using namespace hrw;
+
+// ------------------------------------------------------------------ errors ------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(HRW_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+    } while (0)
+
+#define LAUNCH_CHECK()                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return fail(HRW_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_));  \
+    } while (0)
+
+#define CALL(expr)                                                                        \
+    do {                                                                                  \
+        int rc_ = (expr);                                                                 \
+        if (rc_ != HRW_OK) return rc_;                                                    \
+    } while (0)
+
+static inline hipStream_t S(hrw_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+static bool is_done(double t, double tout, double dt)   // src/hrweno_tvdode.f90:273-284
+{
+    return (t - tout) * std::copysign(1.0, dt) > 0.0;
+}
+
+template <class T> static int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return HRW_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(HRW_ERR_ALLOC, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T),
+                    hipGetErrorString(e));
+    return HRW_OK;
+}
+template <class T> static void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+static unsigned grid_for(long long n, int per_block, unsigned cap = 1u << 20)
+{
+    long long g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    return (unsigned)(g > cap ? cap : g);
+}
+
+// ------------------------------------------------------------------ runtime -----------
+extern "C" const char *hrw_version(void) { return "hrweno-mi355x 0.1.0 (gfx950)"; }
+extern "C" const char *hrw_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hrw_device_count(int *count)
+{
+    HIP_TRY(hipGetDeviceCount(count));
+    return HRW_OK;
+}
+extern "C" int hrw_device_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return fail(HRW_ERR_ARG, "hrw_device_alloc: null output pointer");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess)
+        return fail(HRW_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return HRW_OK;
+}
+extern "C" int hrw_device_free(void *ptr)
+{
+    HIP_TRY(hipFree(ptr));
+    return HRW_OK;
+}
+extern "C" int hrw_memcpy_htod(void *dst, const void *src, size_t bytes, hrw_stream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s)));
+    HIP_TRY(hipStreamSynchronize(S(s)));
+    return HRW_OK;
+}
+extern "C" int hrw_memcpy_dtoh(void *dst, const void *src, size_t bytes, hrw_stream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(s)));
+    HIP_TRY(hipStreamSynchronize(S(s)));
+    return HRW_OK;
+}
+extern "C" int hrw_memcpy_dtod(void *dst, const void *src, size_t bytes, hrw_stream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(s)));
+    return HRW_OK;
+}
+extern "C" int hrw_stream_create(hrw_stream_t *stream)
+{
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = reinterpret_cast<hrw_stream_t>(s);
+    return HRW_OK;
+}
+extern "C" int hrw_stream_destroy(hrw_stream_t stream)
+{
+    HIP_TRY(hipStreamDestroy(S(stream)));
+    return HRW_OK;
+}
+extern "C" int hrw_stream_synchronize(hrw_stream_t stream)
+{
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return HRW_OK;
+}
+
+static thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
+extern "C" int hrw_timer_start(hrw_stream_t stream)
+{
+    for (auto &e : g_ev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(g_ev[0], S(stream)));
+    return HRW_OK;
+}
+extern "C" int hrw_timer_stop(hrw_stream_t stream, double *ms)
+{
+    if (!g_ev[0]) return fail(HRW_ERR_STATE, "hrw_timer_stop without hrw_timer_start");
+    HIP_TRY(hipEventRecord(g_ev[1], S(stream)));
+    HIP_TRY(hipEventSynchronize(g_ev[1]));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, g_ev[0], g_ev[1]));
+    *ms = f;
+    return HRW_OK;
+}
+
+// ------------------------------------------------------------------ grids -------------
+// src/hrweno_grids.f90:41-230 (setup arithmetic, evaluated once on the host).
+extern "C" int hrw_grid_linear(double xmin, double xmax, int n, double *edges)
+{
+    if (xmax <= xmin) return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin.");
+    if (n < 1) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 1.");
+    const double rx = (xmax - xmin) / n;
+    for (int i = 0; i <= n; ++i) edges[i] = linear_edge(xmin, rx, i);
+    return HRW_OK;
+}
+extern "C" int hrw_grid_bilinear(double xmin, double xcross, double xmax, int n1, int n2,
+                                 double *edges)
+{
+    if (xcross <= xmin)
+        return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xcross'. Valid range: xcross > xmin.");
+    if (xmax <= xcross)
+        return fail(HRW_ERR_ARG, "Invalid input 'xcross', 'xmax'. Valid range: xmax > xcross.");
+    if (n1 < 1 || n2 < 1)
+        return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells(i) >= 1.");
+    double rx = (xcross - xmin) / n1;
+    for (int i = 0; i <= n1; ++i) edges[i] = linear_edge(xmin, rx, i);
+    rx = (xmax - xcross) / n2;
+    for (int i = 1; i <= n2; ++i) edges[n1 + i] = linear_edge(xcross, rx, i);
+    return HRW_OK;
+}
+extern "C" int hrw_grid_log(double xmin, double xmax, int n, double *edges)
+{
+    if (xmin <= 0.0) return fail(HRW_ERR_ARG, "Invalid input 'xmin'. Valid range: xmin > 0.");
+    if (!(xmax > xmin)) return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin.");
+    if (n < 1) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 1.");
+    const double rx = (std::log(xmax / xmin)) / n;
+    const double l0 = std::log(xmin);
+    for (int i = 0; i <= n; ++i) edges[i] = linear_edge(l0, rx, i);
+    for (int i = 0; i <= n; ++i) edges[i] = std::exp(edges[i]);
+    return HRW_OK;
+}
+static double powi(double x, int m)   // real**integer by binary powering
+{
+    unsigned e = m < 0 ? -(unsigned)m : (unsigned)m;
+    double y = (e % 2) ? x : 1.0;
+    while (e >>= 1) {
+        x = x * x;
+        if (e % 2) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+extern "C" int hrw_grid_geometric(double xmin, double xmax, double ratio, int n, double *edges)
+{
+    if (xmax <= xmin) return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin");
+    if (ratio <= 0.0) return fail(HRW_ERR_ARG, "Invalid input 'ratio'. Valid range: ratio > 0");
+    if (n < 1) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 1");
+    const double a = (xmax - xmin) / (powi(ratio, n) - 1.0);
+    for (int i = 0; i <= n; ++i) {
+        const double p = powi(ratio, i) - 1;
+        const double ap = a * p;
+        edges[i] = xmin + ap;
+    }
+    return HRW_OK;
+}
+extern "C" int hrw_grid_compute(int n, const double *edges, double *center, double *width)
+{
+    if (n < 1) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 1.");
+    for (int i = 0; i < n; ++i) {
+        if (center) center[i] = (edges[i] + edges[i + 1]) / 2;
+        if (width) width[i] = edges[i + 1] - edges[i];
+    }
+    return HRW_OK;
+}
+
+// ------------------------------------------------------------------ WENO --------------
+struct hrw_weno_s {
+    int n = 0, k = 3, arith = HRW_ARITH_FAST;
+    double eps = 1e-6;
+    bool uniform = true;
+    double *d_cnu = nullptr;                 // (n, k+1, k) on the device
+    double *stage[3] = {nullptr, nullptr, nullptr};   // host-array staging
+    int64_t stage_n = 0;
+    ~hrw_weno_s()
+    {
+        dfree(d_cnu);
+        for (auto &p : stage) dfree(p);
+    }
+};
+
+extern "C" int hrw_weno_tables(int k, double *d, double *c)
+{
+    if (k < 1 || k > 3) return fail(HRW_ERR_ARG, "Invalid input 'k'. Valid range: 1 <= k <= 3.");
+    auto put = [&](auto tab) {
+        for (int r = 0; r < k; ++r) d[r] = tab.d[r];
+        for (int r = 0; r <= k; ++r)
+            for (int j = 0; j < k; ++j) c[r * k + j] = tab.c[r][j];
+    };
+    if (k == 1) put(Tab<1>{});
+    else if (k == 2) put(Tab<2>{});
+    else put(Tab<3>{});
+    return HRW_OK;
+}
+
+// cnu(j, r, i), src/hrweno_weno.f90:261-295, one thread per (i, r, j); xl(m)=xext(m-1),
+// xr(m)=xext(m) with xext already extended by k+1 ghost edges per side (offset k+1).
+__global__ void calc_cnu_kernel(int n, int k, const double *xext, double *cnu)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = k * (k + 1);
+    if (t >= (long long)n * per) return;
+    const int i = (int)(t / per) + 1;
+    const int rj = (int)(t % per);
+    const int r = rj / k - 1, j = rj % k;
+    const int ng = k + 1;
+    auto XL = [&](int m) { return xext[m - 1 + ng]; };
+    auto XR = [&](int m) { return xext[m + ng]; };
+    double sum2 = 0.0;
+    for (int m = j + 1; m <= k; ++m) {
+        double prod2 = 1.0;
+        for (int l = 0; l <= k; ++l) {
+            if (l == m) continue;
+            prod2 = prod2 * (XL(i - r + m) - XL(i - r + l));
+        }
+        double sum1 = 0.0;
+        for (int l = 0; l <= k; ++l) {
+            if (l == m) continue;
+            double prod1 = 1.0;
+            for (int q = 0; q <= k; ++q) {
+                if (q == m || q == l) continue;
+                prod1 = prod1 * (XR(i) - XL(i - r + q));
+            }
+            sum1 = sum1 + prod1;
+        }
+        sum2 = sum2 + sum1 / prod2;
+    }
+    cnu[t] = sum2 * (XR(i - r + j) - XL(i - r + j));
+}
+
+extern "C" int hrw_weno_create(int ncells, int k, double eps, const double *xedges, int arith,
+                               hrw_weno_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "hrw_weno_create: null output pointer");
+    *out = nullptr;
+    // src/hrweno_weno.f90:72-98
+    if (!(ncells > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
+    if (!(k >= 1 && k <= 3)) return fail(HRW_ERR_ARG, "Invalid input 'k'. Valid range: 1 <= k <= 3.");
+    if (!(eps > 2.220446049250313e-16))
+        return fail(HRW_ERR_ARG, "Invalid input 'eps'. Valid range: eps > epsilon.");
+    if (arith != HRW_ARITH_EXACT && arith != HRW_ARITH_FAST)
+        return fail(HRW_ERR_ARG, "Invalid arithmetic form %d", arith);
+    auto *w = new (std::nothrow) hrw_weno_s;
+    if (!w) return fail(HRW_ERR_ALLOC, "out of host memory");
+    w->n = ncells;
+    w->k = k;
+    w->eps = eps;
+    w->arith = arith;
+    if (xedges) {   // :100-109
+        w->uniform = false;
+        const int ng = k + 1;
+        std::vector<double> xext(ncells + 1 + 2 * ng);
+        for (int i = 0; i <= ncells; ++i) xext[i + ng] = xedges[i];
+        double dx = xext[1 + ng] - xext[0 + ng];               // :246-249
+        for (int i = -1; i >= -ng; --i) xext[i + ng] = xext[i + 1 + ng] - dx;
+        dx = xext[ncells + ng] - xext[ncells - 1 + ng];        // :252-255
+        for (int i = ncells + 1; i <= ncells + ng; ++i) xext[i + ng] = xext[i - 1 + ng] + dx;
+        double *d_xext = nullptr;
+        int rc = dalloc(&d_xext, xext.size());
+        if (rc == HRW_OK) rc = dalloc(&w->d_cnu, (size_t)ncells * k * (k + 1));
+        if (rc != HRW_OK) {
+            dfree(d_xext);
+            delete w;
+            return rc;
+        }
+        hipError_t e = hipMemcpy(d_xext, xext.data(), xext.size() * sizeof(double),
+                                 hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            const long long tot = (long long)ncells * k * (k + 1);
+            calc_cnu_kernel<<<grid_for(tot, 256, 1u << 30), 256>>>(ncells, k, d_xext, w->d_cnu);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+        }
+        dfree(d_xext);
+        if (e != hipSuccess) {
+            delete w;
+            return fail(HRW_ERR_HIP, "cnu setup failed: %s", hipGetErrorString(e));
+        }
+    }
+    *out = w;
+    return HRW_OK;
+}
+
+extern "C" int hrw_weno_destroy(hrw_weno_t w)
+{
+    delete w;
+    return HRW_OK;
+}
+
+extern "C" int hrw_weno_info(hrw_weno_t w, int *ncells, int *k, double *eps, int *uniform)
+{
+    if (!w) return fail(HRW_ERR_ARG, "null weno object");
+    if (ncells) *ncells = w->n;
+    if (k) *k = w->k;
+    if (eps) *eps = w->eps;
+    if (uniform) *uniform = w->uniform ? 1 : 0;
+    return HRW_OK;
+}
+
+extern "C" int hrw_weno_get_cnu(hrw_weno_t w, double *cnu_host)
+{
+    if (!w) return fail(HRW_ERR_ARG, "null weno object");
+    if (w->uniform) return fail(HRW_ERR_STATE, "uniform grid: no cnu table");
+    HIP_TRY(hipMemcpy(cnu_host, w->d_cnu, (size_t)w->n * w->k * (w->k + 1) * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    return HRW_OK;
+}
+
+template <int K>
+static void launch_reconstruct(const hrw_weno_s *w, int64_t nlines, const double *v, int64_t incv,
+                               int64_t ldv, double *vl, double *vr, int64_t inco, int64_t ldo,
+                               hipStream_t s)
+{
+    dim3 grid(grid_for(w->n, kBlock, 1u << 30), (unsigned)nlines);
+    if (!w->uniform)
+        reconstruct_kernel<K, true, false><<<grid, kBlock, 0, s>>>(v, incv, ldv, w->n, w->eps,
+                                                                  w->d_cnu, vl, vr, inco, ldo);
+    else if (w->arith == HRW_ARITH_FAST)
+        reconstruct_kernel<K, false, true><<<grid, kBlock, 0, s>>>(v, incv, ldv, w->n, w->eps,
+                                                                  nullptr, vl, vr, inco, ldo);
+    else
+        reconstruct_kernel<K, false, false><<<grid, kBlock, 0, s>>>(v, incv, ldv, w->n, w->eps,
+                                                                   nullptr, vl, vr, inco, ldo);
+}
+
+extern "C" int hrw_weno_reconstruct_lines(hrw_weno_t w, int64_t nlines, const double *v,
+                                          int64_t incv, int64_t ldv, double *vl, double *vr,
+                                          int64_t inco, int64_t ldo, hrw_stream_t stream)
+{
+    if (!w) return fail(HRW_ERR_ARG, "null weno object");
+    if (nlines < 1 || nlines > 65535) return fail(HRW_ERR_ARG, "nlines must be in [1, 65535]");
+    if (!v || !vl || !vr) return fail(HRW_ERR_ARG, "null array");
+    if (w->k == 1) launch_reconstruct<1>(w, nlines, v, incv, ldv, vl, vr, inco, ldo, S(stream));
+    else if (w->k == 2) launch_reconstruct<2>(w, nlines, v, incv, ldv, vl, vr, inco, ldo, S(stream));
+    else launch_reconstruct<3>(w, nlines, v, incv, ldv, vl, vr, inco, ldo, S(stream));
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+extern "C" int hrw_weno_reconstruct(hrw_weno_t w, const double *v, int64_t incv, double *vl,
+                                    int64_t incl, double *vr, int64_t incr, hrw_stream_t stream)
+{
+    if (!w) return fail(HRW_ERR_ARG, "null weno object");
+    if (incl != incr) {
+        // different output strides: reconstruct into vl/vr via two launches of the same kernel
+        // would double the work; require equal strides (contiguous outputs are the norm).
+        return fail(HRW_ERR_ARG, "vl and vr must have the same stride");
+    }
+    return hrw_weno_reconstruct_lines(w, 1, v, incv, 0, vl, vr, incl, 0, stream);
+}
+
+extern "C" int hrw_weno_reconstruct_host(hrw_weno_t w, const double *v, int64_t incv,
+                                         double *vl, double *vr)
+{
+    if (!w) return fail(HRW_ERR_ARG, "null weno object");
+    const int64_t n = w->n;
+    if (w->stage_n != n) {
+        for (auto &p : w->stage) dfree(p);
+        for (auto &p : w->stage) CALL(dalloc(&p, n));
+        w->stage_n = n;
+    }
+    if (incv == 1) {
+        HIP_TRY(hipMemcpy(w->stage[0], v, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy2D(w->stage[0], sizeof(double), v, incv * sizeof(double), sizeof(double),
+                            n, hipMemcpyHostToDevice));
+    }
+    CALL(hrw_weno_reconstruct_lines(w, 1, w->stage[0], 1, 0, w->stage[1], w->stage[2], 1, 0,
+                                    nullptr));
+    HIP_TRY(hipMemcpy(vl, w->stage[1], n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vr, w->stage[2], n * sizeof(double), hipMemcpyDeviceToHost));
+    return HRW_OK;
+}
+
+// ------------------------------------------------------------------ fluxes ------------
+static int flux_common(int which, int fn, double a, double b, int64_t n, const double *vm,
+                       const double *vp, const double *x, double t, double alpha, double *h,
+                       hrw_stream_t stream)
+{
+    if (fn != HRW_FN_BURGERS && fn != HRW_FN_LINEAR) return fail(HRW_ERR_ARG, "unknown flux function %d", fn);
+    if (n < 0) return fail(HRW_ERR_ARG, "negative size");
+    if (n == 0) return HRW_OK;
+    flux_kernel<<<grid_for(n, kBlock, 65536), kBlock, 0, S(stream)>>>(which, fn, a, b, n, vm, vp,
+                                                                       x, t, alpha, h);
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+extern "C" int hrw_flux_lax_friedrichs(int fn, double a, double b, int64_t n, const double *vm,
+                                       const double *vp, const double *x, double t, double alpha,
+                                       double *h, hrw_stream_t stream)
+{
+    return flux_common(0, fn, a, b, n, vm, vp, x, t, alpha, h, stream);
+}
+extern "C" int hrw_flux_godunov(int fn, double a, double b, int64_t n, const double *vm,
+                                const double *vp, const double *x, double t, double *h,
+                                hrw_stream_t stream)
+{
+    return flux_common(1, fn, a, b, n, vm, vp, x, t, 0.0, h, stream);
+}
+
+// ------------------------------------------------------------------ problems ----------
+enum { PROB_BURGERS1D = 1, PROB_PBE2D = 2 };
+
+struct hrw_problem_s {
+    int kind = 0;
+    int64_t neq = 0;
+    int k = 3, arith = HRW_ARITH_FAST, numflux = HRW_FLUX_LAX_FRIEDRICHS, bc = HRW_BC_PERIODIC;
+    double eps = 1e-6, alpha = -1.0;
+    // 1D
+    double xmin = 0, rx = 0;
+    double *d_width = nullptr;
+    // 2D
+    int n1 = 0, n2 = 0, ty = 16;
+    double x1min = 0, rx1 = 0, x2min = 0, rx2 = 0, g1a = 0, g1b = 0, g2a = 0, g2b = 0;
+    // alpha slot sets for rhs-only calls
+    unsigned long long *slots = nullptr;
+    bool alphamax() const { return kind == PROB_BURGERS1D && numflux == HRW_FLUX_LAX_FRIEDRICHS && alpha < 0; }
+    ~hrw_problem_s()
+    {
+        dfree(d_width);
+        dfree(slots);
+    }
+};
+
+extern "C" int hrw_problem_burgers1d(int n, int k, double eps, int numflux, int bc, double alpha,
+                                     double xmin, double xmax, const double *width, int arith,
+                                     hrw_problem_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!(n > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
+    if (!(k >= 1 && k <= 3)) return fail(HRW_ERR_ARG, "Invalid input 'k'. Valid range: 1 <= k <= 3.");
+    if (!(eps > 2.220446049250313e-16))
+        return fail(HRW_ERR_ARG, "Invalid input 'eps'. Valid range: eps > epsilon.");
+    if (numflux != HRW_FLUX_LAX_FRIEDRICHS && numflux != HRW_FLUX_GODUNOV)
+        return fail(HRW_ERR_ARG, "unknown numerical flux %d", numflux);
+    if (bc != HRW_BC_PERIODIC && bc != HRW_BC_ZERO_GRADIENT)
+        return fail(HRW_ERR_ARG, "1D Burgers supports HRW_BC_PERIODIC and HRW_BC_ZERO_GRADIENT");
+    if (arith != HRW_ARITH_EXACT && arith != HRW_ARITH_FAST)
+        return fail(HRW_ERR_ARG, "Invalid arithmetic form %d", arith);
+    if (!width && !(xmax > xmin))
+        return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin.");
+    auto *p = new (std::nothrow) hrw_problem_s;
+    if (!p) return fail(HRW_ERR_ALLOC, "out of host memory");
+    p->kind = PROB_BURGERS1D;
+    p->neq = n;
+    p->k = k;
+    p->eps = eps;
+    p->numflux = numflux;
+    p->bc = bc;
+    p->alpha = alpha;
+    p->arith = arith;
+    p->xmin = xmin;
+    p->rx = (xmax - xmin) / n;   // src/hrweno_grids.f90:76
+    int rc = HRW_OK;
+    if (width) {
+        rc = dalloc(&p->d_width, (size_t)n);
+        if (rc == HRW_OK && hipMemcpy(p->d_width, width, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(HRW_ERR_HIP, "width upload failed");
+    }
+    if (rc == HRW_OK) rc = dalloc(&p->slots, (size_t)3 * kNSlot * kSlotStride);
+    if (rc != HRW_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return HRW_OK;
+}
+
+extern "C" int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min,
+                                 double x1max, double x2min, double x2max, double g1a, double g1b,
+                                 double g2a, double g2b, int arith, hrw_problem_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!(n1 > 0 && n2 > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
+    if ((long long)n1 * n2 > (1ll << 31) - 1) return fail(HRW_ERR_ARG, "n1*n2 must fit a 32-bit integer");
+    if (!(k >= 1 && k <= 3)) return fail(HRW_ERR_ARG, "Invalid input 'k'. Valid range: 1 <= k <= 3.");
+    if (!(eps > 2.220446049250313e-16))
+        return fail(HRW_ERR_ARG, "Invalid input 'eps'. Valid range: eps > epsilon.");
+    if (bc != HRW_BC_ZERO_FLUX && bc != HRW_BC_INFLOW0_OUTFLOW)
+        return fail(HRW_ERR_ARG, "2D PBE supports HRW_BC_ZERO_FLUX and HRW_BC_INFLOW0_OUTFLOW");
+    if (!(x1max > x1min) || !(x2max > x2min))
+        return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin.");
+    if (arith != HRW_ARITH_EXACT && arith != HRW_ARITH_FAST)
+        return fail(HRW_ERR_ARG, "Invalid arithmetic form %d", arith);
+    auto *p = new (std::nothrow) hrw_problem_s;
+    if (!p) return fail(HRW_ERR_ALLOC, "out of host memory");
+    p->kind = PROB_PBE2D;
+    p->neq = (int64_t)n1 * n2;
+    p->n1 = n1;
+    p->n2 = n2;
+    p->k = k;
+    p->eps = eps;
+    p->bc = bc;
+    p->numflux = HRW_FLUX_GODUNOV;
+    p->arith = arith;
+    p->x1min = x1min;
+    p->rx1 = (x1max - x1min) / n1;
+    p->x2min = x2min;
+    p->rx2 = (x2max - x2min) / n2;
+    p->g1a = g1a;
+    p->g1b = g1b;
+    p->g2a = g2a;
+    p->g2b = g2b;
+    *out = p;
+    return HRW_OK;
+}
+
+extern "C" int hrw_problem_destroy(hrw_problem_t p)
+{
+    delete p;
+    return HRW_OK;
+}
+extern "C" int hrw_problem_neq(hrw_problem_t p, int64_t *neq)
+{
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    *neq = p->neq;
+    return HRW_OK;
+}
+
+// ---- kernel dispatch ----------------------------------------------------------------
+constexpr int kCPT = 4;   // cells per thread in the 1D stage kernel
+
+template <int K, int FLUX, bool PER, bool FAST, bool AMAX>
+static void l1d(const Stage1DArgs &a, hipStream_t s)
+{
+    constexpr int TILE = kBlock * kCPT - 2;
+    const unsigned grid = (unsigned)((a.n + TILE - 1) / TILE);
+    stage1d_kernel<K, FLUX, PER, FAST, AMAX, kCPT><<<grid, kBlock, 0, s>>>(a);
+}
+template <int K, int FLUX, bool PER, bool FAST>
+static void l1d_a(bool amax, const Stage1DArgs &a, hipStream_t s)
+{
+    if constexpr (FLUX == LF) {
+        if (amax) return l1d<K, FLUX, PER, FAST, true>(a, s);
+    }
+    l1d<K, FLUX, PER, FAST, false>(a, s);
+}
+template <int K, int FLUX, bool PER>
+static void l1d_f(bool fast, bool amax, const Stage1DArgs &a, hipStream_t s)
+{
+    if (fast) l1d_a<K, FLUX, PER, true>(amax, a, s);
+    else l1d_a<K, FLUX, PER, false>(amax, a, s);
+}
+template <int K, int FLUX>
+static void l1d_p(bool per, bool fast, bool amax, const Stage1DArgs &a, hipStream_t s)
+{
+    if (per) l1d_f<K, FLUX, true>(fast, amax, a, s);
+    else l1d_f<K, FLUX, false>(fast, amax, a, s);
+}
+template <int K>
+static void l1d_x(int flux, bool per, bool fast, bool amax, const Stage1DArgs &a, hipStream_t s)
+{
+    if (flux == LF) l1d_p<K, LF>(per, fast, amax, a, s);
+    else l1d_p<K, GODUNOV>(per, fast, amax, a, s);
+}
+static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
+{
+    const bool per = p->bc == HRW_BC_PERIODIC, fast = p->arith == HRW_ARITH_FAST;
+    const bool amax = p->alphamax();
+    if (p->k == 1) l1d_x<1>(p->numflux, per, fast, amax, a, s);
+    else if (p->k == 2) l1d_x<2>(p->numflux, per, fast, amax, a, s);
+    else l1d_x<3>(p->numflux, per, fast, amax, a, s);
+}
+
+template <int K, int BC, bool FAST>
+static void l2d(const Stage2DArgs &a, hipStream_t s)
+{
+    const long long nsy = (a.n2 + a.ty - 1) / a.ty;
+    const long long waves = nsy * a.nstrips;
+    const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
+    pbe2d_kernel<K, BC, FAST><<<grid, kBlock, 0, s>>>(a);
+}
+template <int K>
+static void l2d_b(int bc, bool fast, const Stage2DArgs &a, hipStream_t s)
+{
+    if (bc == HRW_BC_ZERO_FLUX) {
+        if (fast) l2d<K, 0, true>(a, s);
+        else l2d<K, 0, false>(a, s);
+    } else {
+        if (fast) l2d<K, 1, true>(a, s);
+        else l2d<K, 1, false>(a, s);
+    }
+}
+static void launch_stage2d(const hrw_problem_s *p, const Stage2DArgs &a, hipStream_t s)
+{
+    const bool fast = p->arith == HRW_ARITH_FAST;
+    if (p->k == 1) l2d_b<1>(p->bc, fast, a, s);
+    else if (p->k == 2) l2d_b<2>(p->bc, fast, a, s);
+    else l2d_b<3>(p->bc, fast, a, s);
+}
+
+// One fused pass: out = stage(us, u0, L(us)) (and/or Lout = L(us)).  `seq` indexes the
+// alpha slot sets when alpha = max|v|.
+struct StageIO {
+    const double *us = nullptr, *u0 = nullptr, *uo4 = nullptr, *Lo4 = nullptr;
+    double *uo = nullptr, *Lout = nullptr;
+};
+static int run_stage(const hrw_problem_s *p, unsigned long long *slots, int seq, int stage,
+                     double h, const StageIO &io, hipStream_t s)
+{
+    if (p->kind == PROB_BURGERS1D) {
+        Stage1DArgs a{};
+        a.us = io.us;
+        a.u0 = io.u0;
+        a.uo = io.uo;
+        a.uo4 = io.uo4;
+        a.Lo4 = io.Lo4;
+        a.Lout = io.Lout;
+        a.width = p->d_width;
+        a.xmin = p->xmin;
+        a.rx = p->rx;
+        a.n = p->neq;
+        a.eps = p->eps;
+        a.h = h;
+        a.alpha = p->alpha;
+        a.stage = stage;
+        if (p->alphamax()) {
+            const size_t set = (size_t)kNSlot * kSlotStride;
+            a.alpha_in = slots + (seq % 3) * set;
+            a.alpha_out = slots + ((seq + 1) % 3) * set;
+            a.alpha_clear = slots + ((seq + 2) % 3) * set;
+        }
+        launch_stage1d(p, a, s);
+    } else {
+        Stage2DArgs a{};
+        a.us = io.us;
+        a.u0 = io.u0;
+        a.uo = io.uo;
+        a.uo4 = io.uo4;
+        a.Lo4 = io.Lo4;
+        a.Lout = io.Lout;
+        a.n1 = p->n1;
+        a.n2 = p->n2;
+        a.x1min = p->x1min;
+        a.rx1 = p->rx1;
+        a.x2min = p->x2min;
+        a.rx2 = p->rx2;
+        a.g1a = p->g1a;
+        a.g1b = p->g1b;
+        a.g2a = p->g2a;
+        a.g2b = p->g2b;
+        a.eps = p->eps;
+        a.h = h;
+        a.stage = stage;
+        a.ty = p->ty;
+        a.nstrips = (p->n1 + kStrip - 1) / kStrip;
+        launch_stage2d(p, a, s);
+    }
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+// Seed alpha = max|u| into slot set 0 (all sets cleared first).
+static int seed_alpha(const hrw_problem_s *p, unsigned long long *slots, const double *u,
+                      hipStream_t s)
+{
+    if (!p->alphamax()) return HRW_OK;
+    clear_slots_kernel<<<grid_for(3 * kNSlot, 256), 256, 0, s>>>(slots, 3);
+    maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, s>>>(u, p->neq, slots);
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+extern "C" int hrw_problem_rhs(hrw_problem_t p, double t, const double *u, double *udot,
+                               hrw_stream_t stream)
+{
+    (void)t;
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    if (!u || !udot) return fail(HRW_ERR_ARG, "null array");
+    CALL(seed_alpha(p, p->slots, u, S(stream)));
+    StageIO io;
+    io.us = u;
+    io.Lout = udot;
+    return run_stage(p, p->slots, 0, -1, 0.0, io, S(stream));
+}
+
+// ------------------------------------------------------------------ integrators -------
+enum { METHOD_RK = 1, METHOD_MS = 2 };
+
+struct hrw_ode_s {
+    int method = METHOD_RK, order = 3, fevals = 0, istate = 0, arith = HRW_ARITH_FAST;
+    int64_t neq = 0;
+    hrw_problem_s *p = nullptr;           // fused device problem, or
+    hrw_integrand_fn fu = nullptr;        // a callback integrand
+    void *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    double *ui = nullptr, *ui2 = nullptr, *udot = nullptr;
+    double *S = nullptr;                   // mstvd: current state
+    double *ring_u[4] = {}, *ring_L[4] = {};
+    int head = 0;                          // ring index of column 4 (oldest)
+    unsigned long long *slots = nullptr;
+    ~hrw_ode_s()
+    {
+        dfree(ui);
+        dfree(ui2);
+        dfree(udot);
+        dfree(S);
+        for (auto &b : ring_u) dfree(b);
+        for (auto &b : ring_L) dfree(b);
+        dfree(slots);
+    }
+};
+
+static int ode_alloc(hrw_ode_s *o)
+{
+    CALL(dalloc(&o->ui, o->neq));
+    CALL(dalloc(&o->ui2, o->neq));
+    CALL(dalloc(&o->udot, o->neq));
+    if (o->method == METHOD_MS) {
+        CALL(dalloc(&o->S, o->neq));
+        for (auto &b : o->ring_u) CALL(dalloc(&b, o->neq));
+        for (auto &b : o->ring_L) CALL(dalloc(&b, o->neq));
+    }
+    CALL(dalloc(&o->slots, (size_t)3 * kNSlot * kSlotStride));
+    return HRW_OK;
+}
+
+static int ode_new(int method, int order, hrw_problem_s *p, hrw_integrand_fn fu, void *ctx,
+                   int64_t neq, int arith, hrw_ode_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!(neq > 0)) return fail(HRW_ERR_ARG, "Invalid input 'neq'. Valid range: neq >= 1.");
+    if (method == METHOD_RK && !(order >= 1 && order <= 3))
+        return fail(HRW_ERR_ARG, "Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.");
+    auto *o = new (std::nothrow) hrw_ode_s;
+    if (!o) return fail(HRW_ERR_ALLOC, "out of host memory");
+    o->method = method;
+    o->order = method == METHOD_MS ? 3 : order;
+    o->p = p;
+    o->fu = fu;
+    o->ctx = ctx;
+    o->neq = neq;
+    o->arith = arith;
+    int rc = ode_alloc(o);
+    if (rc != HRW_OK) {
+        delete o;
+        return rc;
+    }
+    o->istate = 1;   // :93, :199
+    *out = o;
+    return HRW_OK;
+}
+
+extern "C" int hrw_rktvd_create(hrw_problem_t p, int order, hrw_ode_t *out)
+{
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    return ode_new(METHOD_RK, order, p, nullptr, nullptr, p->neq, p->arith, out);
+}
+extern "C" int hrw_rktvd_create_cb(hrw_integrand_fn fu, void *ctx, int64_t neq, int order,
+                                   int arith, hrw_ode_t *out)
+{
+    if (!fu) return fail(HRW_ERR_ARG, "null integrand");
+    return ode_new(METHOD_RK, order, nullptr, fu, ctx, neq, arith, out);
+}
+extern "C" int hrw_mstvd_create(hrw_problem_t p, hrw_ode_t *out)
+{
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    return ode_new(METHOD_MS, 3, p, nullptr, nullptr, p->neq, p->arith, out);
+}
+extern "C" int hrw_mstvd_create_cb(hrw_integrand_fn fu, void *ctx, int64_t neq, int arith,
+                                   hrw_ode_t *out)
+{
+    if (!fu) return fail(HRW_ERR_ARG, "null integrand");
+    return ode_new(METHOD_MS, 3, nullptr, fu, ctx, neq, arith, out);
+}
+extern "C" int hrw_ode_destroy(hrw_ode_t ode)
+{
+    delete ode;
+    return HRW_OK;
+}
+extern "C" int hrw_ode_set_stream(hrw_ode_t ode, hrw_stream_t stream)
+{
+    if (!ode) return fail(HRW_ERR_ARG, "null ode");
+    ode->stream = S(stream);
+    return HRW_OK;
+}
+extern "C" int hrw_ode_info(hrw_ode_t ode, int64_t *neq, int *order, int *fevals, int *istate)
+{
+    if (!ode) return fail(HRW_ERR_ARG, "null ode");
+    if (neq) *neq = ode->neq;
+    if (order) *order = ode->order;
+    if (fevals) *fevals = ode->fevals;
+    if (istate) *istate = ode->istate;
+    return HRW_OK;
+}
+
+// L(us) -> Lout through the callback, or a fused rhs-only pass.
+static int ode_rhs(hrw_ode_s *o, int &seq, double t, const double *us, double *Lout)
+{
+    if (o->fu) {
+        int rc = o->fu(o->ctx, t, us, Lout, reinterpret_cast<hrw_stream_t>(o->stream));
+        if (rc != 0) return fail(HRW_ERR_STATE, "integrand callback returned %d", rc);
+        return HRW_OK;
+    }
+    StageIO io;
+    io.us = us;
+    io.Lout = Lout;
+    CALL(run_stage(o->p, o->slots, seq, -1, 0.0, io, o->stream));
+    ++seq;
+    return HRW_OK;
+}
+
+// stage(us, u0, L) -> uo; fused (L computed in the same pass) or callback + combine.
+static int ode_stage(hrw_ode_s *o, int &seq, int stage, double t, double h, const double *us,
+                     const double *u0, double *uo, const double *uo4 = nullptr,
+                     const double *Lo4 = nullptr, double *Lout = nullptr)
+{
+    if (o->fu) {
+        double *L = Lout ? Lout : o->udot;
+        CALL(ode_rhs(o, seq, t, us, L));
+        const unsigned g = grid_for(o->neq, kBlock * 4, 8192);
+        if (o->arith == HRW_ARITH_FAST)
+            combine_kernel<true><<<g, kBlock, 0, o->stream>>>(stage, o->neq, h, u0, us, L, uo4, Lo4, uo);
+        else
+            combine_kernel<false><<<g, kBlock, 0, o->stream>>>(stage, o->neq, h, u0, us, L, uo4, Lo4, uo);
+        LAUNCH_CHECK();
+        return HRW_OK;
+    }
+    StageIO io;
+    io.us = us;
+    io.u0 = u0;
+    io.uo = uo;
+    io.uo4 = uo4;
+    io.Lo4 = Lo4;
+    io.Lout = Lout;
+    CALL(run_stage(o->p, o->slots, seq, stage, h, io, o->stream));
+    ++seq;
+    return HRW_OK;
+}
+
+// One RK step of order o->order on state u (in place for orders 2, 3).  For order 1 the new
+// state is written to `other` and the caller swaps.
+static int rk_step(hrw_ode_s *o, int &seq, double t, double dt, double *u, double *other)
+{
+    switch (o->order) {
+    case 1:   // :138-139
+        CALL(ode_stage(o, seq, RK1, t, dt, u, u, other));
+        break;
+    case 2:   // :149-152
+        CALL(ode_stage(o, seq, RK2_1, t, dt, u, u, o->ui));
+        CALL(ode_stage(o, seq, RK2_2, t + dt, dt, o->ui, u, u));
+        break;
+    default:  // :162-167
+        CALL(ode_stage(o, seq, RK3_1, t, dt, u, u, o->ui));
+        CALL(ode_stage(o, seq, RK3_2, t + dt, dt, o->ui, u, o->ui2));
+        CALL(ode_stage(o, seq, RK3_3, t + dt / 2, dt, o->ui2, u, u));
+        break;
+    }
+    return HRW_OK;
+}
+
+static int rk_integrate(hrw_ode_s *o, double *u, double *t, double tout, double dt, int itask)
+{
+    if (o->istate < 1) return HRW_OK;                  // :126
+    if (is_done(*t, tout, dt)) return HRW_OK;          // :127
+    int seq = 0;
+    if (o->p) CALL(seed_alpha(o->p, o->slots, u, o->stream));
+    double *cur = u, *other = o->ui2;
+    for (;;) {                                         // :137-171
+        if (o->order == 1) {
+            CALL(rk_step(o, seq, *t, dt, cur, other));
+            std::swap(cur, other);
+        } else {
+            CALL(rk_step(o, seq, *t, dt, cur, nullptr));
+        }
+        *t = *t + dt;
+        o->fevals += o->order;
+        if (is_done(*t, tout, dt) || itask == 2) break;
+    }
+    if (cur != u) HIP_TRY(hipMemcpyAsync(u, cur, o->neq * sizeof(double), hipMemcpyDeviceToDevice, o->stream));
+    if (o->istate == 1) o->istate = 2;                 // :176
+    return HRW_OK;
+}
+
+static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double dt)
+{
+    if (o->istate < 1) return HRW_OK;                  // :228
+    if (is_done(*t, tout, dt)) return HRW_OK;          // :229
+    const size_t bytes = o->neq * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(o->S, u, bytes, hipMemcpyDeviceToDevice, o->stream));
+    int seq = 0;
+    if (o->p) CALL(seed_alpha(o->p, o->slots, o->S, o->stream));
+    auto col = [o](int c) { return (o->head + 4 - c) % 4; };   // column c (1 newest .. 4 oldest)
+    if (o->istate == 1) {                              // :236-249, 4 RK3 starting steps
+        int fe = 0;
+        for (int c = 4; c >= 1; --c) {
+            HIP_TRY(hipMemcpyAsync(o->ring_u[col(c)], o->S, bytes, hipMemcpyDeviceToDevice, o->stream));
+            CALL(ode_rhs(o, seq, *t, o->S, o->ring_L[col(c)]));
+            CALL(rk_step(o, seq, *t, dt, o->S, nullptr));   // order 3
+            *t = *t + dt;
+            fe += 3;
+        }
+        o->fevals = fe;
+        o->istate = 2;
+    }
+    for (;;) {                                         // :252-268
+        if (is_done(*t, tout, dt)) break;
+        const int h4 = o->head;
+        if (o->fu) {
+            // callback: L(u_n) lands in udot, then takes the place of L(u_{n-4})
+            CALL(ode_stage(o, seq, MS3, *t, dt, o->S, o->S, o->ring_u[h4], o->ring_u[h4],
+                           o->ring_L[h4], nullptr));
+            std::swap(o->ring_L[h4], o->udot);
+        } else {
+            // fused: u_{n+1} overwrites u_{n-4} and L(u_n) overwrites L(u_{n-4}) pointwise
+            CALL(ode_stage(o, seq, MS3, *t, dt, o->S, o->S, o->ring_u[h4], o->ring_u[h4],
+                           o->ring_L[h4], o->ring_L[h4]));
+        }
+        *t = *t + dt;
+        o->fevals += 1;
+        std::swap(o->S, o->ring_u[h4]);                // column 1 <- u_n, state <- u_{n+1}
+        o->head = (h4 + 1) % 4;
+    }
+    HIP_TRY(hipMemcpyAsync(u, o->S, bytes, hipMemcpyDeviceToDevice, o->stream));
+    return HRW_OK;
+}
+
+extern "C" int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tout, double dt,
+                                 int itask)
+{
+    if (!ode) return fail(HRW_ERR_ARG, "null ode");
+    if (!u || !t) return fail(HRW_ERR_ARG, "null argument");
+    if (itask != 1 && itask != 2) return fail(HRW_ERR_ARG, "Invalid input 'itask'. Valid values: 1, 2.");
+    // the reference loops forever on dt == 0 (is_done never turns true); refuse it instead
+    if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(*t) || std::isnan(tout))
+        return fail(HRW_ERR_ARG, "Invalid input 'dt', 't' or 'tout': dt must be finite and nonzero");
+    int rc = ode->method == METHOD_MS ? ms_integrate(ode, u, t, tout, dt)
+                                      : rk_integrate(ode, u, t, tout, dt, itask);
+    if (rc != HRW_OK) ode->istate = -1;
+    return rc;
+}

@@ -1,0 +1,340 @@
+"""Python mirror of the reference's Fortran API over the gfx950 C-ABI.
+
+Names, argument meanings and error texts follow the reference modules
+(src/hrweno_weno.f90, src/hrweno_fluxes.f90, src/hrweno_grids.f90, src/hrweno_tvdode.f90).
+Device arrays are torch float64 CUDA tensors (any 1D stride for ``reconstruct``); work is
+enqueued on torch's current stream, so torch events time it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import HrwError, call
+
+EXACT, FAST = 0, 1
+LAX_FRIEDRICHS, GODUNOV = 0, 1
+FN_BURGERS, FN_LINEAR = 0, 1
+BC_PERIODIC, BC_ZERO_GRADIENT, BC_ZERO_FLUX, BC_INFLOW0_OUTFLOW = 0, 1, 2, 3
+
+_ARITH = {"exact": EXACT, "fast": FAST, EXACT: EXACT, FAST: FAST}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t) -> C.c_void_p:
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError("expected a float64 CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _np(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(_lib.PD)
+
+
+def version() -> str:
+    return _lib.lib().hrw_version().decode()
+
+
+# ---------------------------------------------------------------------------------- weno
+def tables(k: int):
+    """d(0:k-1) and c(0:k-1,-1:k-1) of src/hrweno_weno.f90:12-21; c is returned as c[j, r+1]."""
+    d = np.zeros(3)
+    c = np.zeros(12)
+    call("hrw_weno_tables", k, _pd(d), _pd(c))
+    return d[:k].copy(), c[: k * (k + 1)].reshape(k + 1, k).T.copy()
+
+
+class Weno:
+    """``weno(ncells, k, eps, xedges)`` (src/hrweno_weno.f90:23-127)."""
+
+    def __init__(self, ncells: int, k: int = 3, eps: float = 1e-6, xedges=None,
+                 arith="fast"):
+        h = C.c_void_p()
+        xe = None
+        if xedges is not None:
+            xe = _np(xedges)
+            if xe.size != ncells + 1:
+                raise HrwError(1, "Invalid input 'xedges': size(xedges) /= ncells + 1.")
+        call("hrw_weno_create", int(ncells), int(k), float(eps),
+             _pd(xe) if xe is not None else None, _ARITH[arith], C.byref(h))
+        self._h = h
+        self.ncells, self.k, self.eps = int(ncells), int(k), float(eps)
+        self.uniform_grid = xe is None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().hrw_weno_destroy(h)
+            self._h = None
+
+    @property
+    def cnu(self) -> np.ndarray:
+        """cnu(0:k-1,-1:k-1,1:ncells) as an array indexed [i-1, r+1, j]."""
+        k, n = self.k, self.ncells
+        out = np.zeros(n * k * (k + 1))
+        call("hrw_weno_get_cnu", self._h, _pd(out))
+        return out.reshape(n, k + 1, k)
+
+    def reconstruct(self, v, vl=None, vr=None):
+        """``call w%reconstruct(v, vl, vr)`` (src/hrweno_weno.f90:129-219) on device tensors."""
+        torch = _torch()
+        if v.dim() != 1 or v.numel() != self.ncells:
+            raise HrwError(1, "reconstruct: v must be a vector of ncells values")
+        if vl is None:
+            vl = torch.empty(self.ncells, dtype=torch.float64, device=v.device)
+        if vr is None:
+            vr = torch.empty(self.ncells, dtype=torch.float64, device=v.device)
+        call("hrw_weno_reconstruct", self._h, _dptr(v), v.stride(0), _dptr(vl), vl.stride(0),
+             _dptr(vr), vr.stride(0), _stream())
+        return vl, vr
+
+    def reconstruct_lines(self, v, nlines: int, incv: int, ldv: int, vl, vr, inco: int, ldo: int):
+        """All nlines lines of a 2D array in one launch (element i of line l at l*ld + i*inc)."""
+        call("hrw_weno_reconstruct_lines", self._h, int(nlines), _dptr(v), int(incv), int(ldv),
+             _dptr(vl), _dptr(vr), int(inco), int(ldo), _stream())
+        return vl, vr
+
+    def reconstruct_host(self, v):
+        """Host numpy arrays in and out (synchronous, through device staging buffers)."""
+        v = np.asarray(v, dtype=np.float64)
+        if v.ndim != 1 or v.size != self.ncells:
+            raise HrwError(1, "reconstruct: v must be a vector of ncells values")
+        step = v.strides[0] // 8
+        base = v if step > 0 else np.ascontiguousarray(v)
+        if step <= 0:
+            step = 1
+        vl = np.empty(self.ncells)
+        vr = np.empty(self.ncells)
+        call("hrw_weno_reconstruct_host", self._h, base.ctypes.data_as(_lib.PD), step, _pd(vl),
+             _pd(vr))
+        return vl, vr
+
+
+# ---------------------------------------------------------------------------------- fluxes
+def lax_friedrichs(vm, vp, x=None, t=0.0, alpha=1.0, fn=FN_BURGERS, coef=(0.0, 1.0), out=None):
+    """Elementwise src/hrweno_fluxes.f90:22-45 for a built-in flux function."""
+    torch = _torch()
+    if out is None:
+        out = torch.empty_like(vm)
+    call("hrw_flux_lax_friedrichs", fn, float(coef[0]), float(coef[1]), vm.numel(), _dptr(vm),
+         _dptr(vp), _dptr(x) if x is not None else None, float(t), float(alpha), _dptr(out),
+         _stream())
+    return out
+
+
+def godunov(vm, vp, x=None, t=0.0, fn=FN_BURGERS, coef=(0.0, 1.0), out=None):
+    """Elementwise src/hrweno_fluxes.f90:47-76 for a built-in flux function."""
+    torch = _torch()
+    if out is None:
+        out = torch.empty_like(vm)
+    call("hrw_flux_godunov", fn, float(coef[0]), float(coef[1]), vm.numel(), _dptr(vm),
+         _dptr(vp), _dptr(x) if x is not None else None, float(t), _dptr(out), _stream())
+    return out
+
+
+# ---------------------------------------------------------------------------------- grids
+class Grid1:
+    """``type(grid1)`` (src/hrweno_grids.f90:12-37); arrays are host numpy."""
+
+    def __init__(self):
+        self.name = ""
+        self.scale = ""
+        self.ncells = 0
+        self.edges = np.zeros(0)
+        self.center = np.zeros(0)
+        self.width = np.zeros(0)
+
+    @property
+    def left(self):
+        return self.edges[:-1]
+
+    @property
+    def right(self):
+        return self.edges[1:]
+
+    def _compute(self, edges, name):
+        n = edges.size - 1
+        self.ncells = n
+        self.edges = edges
+        self.center = np.zeros(n)
+        self.width = np.zeros(n)
+        call("hrw_grid_compute", n, _pd(edges), _pd(self.center), _pd(self.width))
+        self.name = name
+
+    def linear(self, xmin, xmax, ncells, name=""):
+        e = np.zeros(int(ncells) + 1) if ncells >= 1 else np.zeros(1)
+        call("hrw_grid_linear", float(xmin), float(xmax), int(ncells), _pd(e))
+        self.scale = "linear"
+        self._compute(e, name)
+        return self
+
+    def bilinear(self, xmin, xcross, xmax, ncells, name=""):
+        n1, n2 = int(ncells[0]), int(ncells[1])
+        e = np.zeros(max(n1 + n2, 0) + 1)
+        call("hrw_grid_bilinear", float(xmin), float(xcross), float(xmax), n1, n2, _pd(e))
+        self.scale = "bilinear"
+        self._compute(e, name)
+        return self
+
+    def log(self, xmin, xmax, ncells, name=""):
+        e = np.zeros(max(int(ncells), 0) + 1)
+        call("hrw_grid_log", float(xmin), float(xmax), int(ncells), _pd(e))
+        self.scale = "log"
+        self._compute(e, name)
+        return self
+
+    def geometric(self, xmin, xmax, ratio, ncells, name=""):
+        e = np.zeros(max(int(ncells), 0) + 1)
+        call("hrw_grid_geometric", float(xmin), float(xmax), float(ratio), int(ncells), _pd(e))
+        self.scale = "geometric"
+        self._compute(e, name)
+        return self
+
+
+# ---------------------------------------------------------------------------------- problems
+class Problem:
+    """A device problem descriptor (the user RHS of the reference examples)."""
+
+    _h = None
+
+    def __del__(self):
+        if self._h:
+            _lib.lib().hrw_problem_destroy(self._h)
+            self._h = None
+
+    @property
+    def neq(self) -> int:
+        n = C.c_int64()
+        call("hrw_problem_neq", self._h, C.byref(n))
+        return n.value
+
+    def rhs(self, t, u, udot=None):
+        torch = _torch()
+        if udot is None:
+            udot = torch.empty_like(u)
+        call("hrw_problem_rhs", self._h, float(t), _dptr(u), _dptr(udot), _stream())
+        return udot
+
+
+class Burgers1D(Problem):
+    """1D Burgers u_t + (u**2/2)_x = 0 (example/example1_burgers_1d_fv.f90:72-122)."""
+
+    def __init__(self, n, k=3, eps=1e-6, flux=LAX_FRIEDRICHS, bc=BC_PERIODIC, alpha=-1.0,
+                 xmin=-1.0, xmax=1.0, width=None, arith="fast"):
+        h = C.c_void_p()
+        w = _np(width) if width is not None else None
+        call("hrw_problem_burgers1d", int(n), int(k), float(eps), int(flux), int(bc),
+             float(alpha), float(xmin), float(xmax), _pd(w) if w is not None else None,
+             _ARITH[arith], C.byref(h))
+        self._h = h
+
+
+class PBE2D(Problem):
+    """2D population balance with growth rates g1 = g1a + g1b*x1, g2 = g2a + g2b*x2
+    (example/example2_pbe_2d_fv.f90:73-155); state u((j-1)*n1 + i)."""
+
+    def __init__(self, n1, n2, k=3, eps=1e-6, bc=BC_INFLOW0_OUTFLOW, x1=(0.0, 1.0),
+                 x2=(0.0, 1.0), g1=(1.0, 0.5), g2=(0.5, 0.0), arith="fast"):
+        h = C.c_void_p()
+        call("hrw_problem_pbe2d", int(n1), int(n2), int(k), float(eps), int(bc), float(x1[0]),
+             float(x1[1]), float(x2[0]), float(x2[1]), float(g1[0]), float(g1[1]),
+             float(g2[0]), float(g2[1]), _ARITH[arith], C.byref(h))
+        self._h = h
+
+
+# ---------------------------------------------------------------------------------- ODEs
+class _Ode:
+    _h = None
+    _cb = None
+
+    def __del__(self):
+        if self._h:
+            _lib.lib().hrw_ode_destroy(self._h)
+            self._h = None
+
+    def _wrap(self, fu):
+        def trampoline(ctx, t, u, udot, stream):
+            try:
+                rc = fu(t, u, udot, stream)
+                return int(rc or 0)
+            except Exception:  # noqa: BLE001 -- reported as a non-zero status
+                return 1
+        self._cb = _lib.INTEGRAND(trampoline)
+        return self._cb
+
+    def _info(self):
+        neq, order, fe, st = C.c_int64(), C.c_int(), C.c_int(), C.c_int()
+        call("hrw_ode_info", self._h, C.byref(neq), C.byref(order), C.byref(fe), C.byref(st))
+        return neq.value, order.value, fe.value, st.value
+
+    neq = property(lambda self: self._info()[0])
+    order = property(lambda self: self._info()[1])
+    fevals = property(lambda self: self._info()[2])
+    istate = property(lambda self: self._info()[3])
+
+    def integrate(self, u, t: float, tout: float, dt: float, itask: int = 1) -> float:
+        """``call ode%integrate(u, t, tout, dt[, itask])``: u (device tensor) is advanced in
+        place; returns the new t (src/hrweno_tvdode.f90:97-178, :203-271)."""
+        if u.numel() != self.neq or not u.is_contiguous():
+            raise HrwError(1, "integrate: u must be a contiguous vector of neq values")
+        tt = C.c_double(t)
+        call("hrw_ode_set_stream", self._h, _stream())
+        call("hrw_ode_integrate", self._h, _dptr(u), C.byref(tt), float(tout), float(dt),
+             int(itask))
+        return tt.value
+
+
+class RkTvd(_Ode):
+    """``rktvd(fu, neq, order)`` (src/hrweno_tvdode.f90:69-95).
+
+    ``fu`` is a Problem (fused device path) or a callable ``fu(t, u_ptr, udot_ptr, stream)``
+    on raw device pointers (then ``neq`` is required)."""
+
+    def __init__(self, fu, neq: Optional[int] = None, order: int = 3, arith="fast"):
+        h = C.c_void_p()
+        if isinstance(fu, Problem):
+            call("hrw_rktvd_create", fu._h, int(order), C.byref(h))
+            self._problem = fu
+        else:
+            call("hrw_rktvd_create_cb", self._wrap(fu), None, int(neq), int(order),
+                 _ARITH[arith], C.byref(h))
+        self._h = h
+
+
+class MsTvd(_Ode):
+    """``mstvd(fu, neq)`` (src/hrweno_tvdode.f90:180-201)."""
+
+    def __init__(self, fu, neq: Optional[int] = None, arith="fast"):
+        h = C.c_void_p()
+        if isinstance(fu, Problem):
+            call("hrw_mstvd_create", fu._h, C.byref(h))
+            self._problem = fu
+        else:
+            call("hrw_mstvd_create_cb", self._wrap(fu), None, int(neq), _ARITH[arith],
+                 C.byref(h))
+        self._h = h
+
+
+def is_done(t: float, tout: float, dt: float) -> bool:
+    """src/hrweno_tvdode.f90:273-284."""
+    return (t - tout) * np.copysign(1.0, dt) > 0.0
+
+
+IntegrandFn = Callable[[float, int, int, int], int]

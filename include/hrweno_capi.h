@@ -1,0 +1,172 @@
+/*
+ * hrweno_capi.h -- C-ABI of the MI355X (gfx950) HR-WENO hot path.
+ *
+ * Plain C types only: device arrays are `double*` obtained from hrw_device_alloc (or any HIP
+ * allocation, e.g. a torch tensor's data pointer), sizes are 64-bit, streams are opaque
+ * hipStream_t values (NULL = the default stream).  Every function returns an hrw_status;
+ * on failure hrw_last_error() holds a message for the calling thread.  Invalid arguments
+ * produce the same messages as the reference's `error stop` texts.
+ *
+ * Citations (path:line) are relative to the reference repository HugoMVale/HR-WENO.
+ */
+#ifndef HRWENO_CAPI_H
+#define HRWENO_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *hrw_stream_t;
+typedef struct hrw_weno_s *hrw_weno_t;
+typedef struct hrw_problem_s *hrw_problem_t;
+typedef struct hrw_ode_s *hrw_ode_t;
+typedef struct hrw_batch_s *hrw_batch_t;
+
+enum hrw_status {
+    HRW_OK = 0,
+    HRW_ERR_ARG = 1,     /* invalid argument (reference: error stop)             */
+    HRW_ERR_HIP = 2,     /* HIP runtime or launch failure                        */
+    HRW_ERR_ALLOC = 3,   /* device allocation failed                             */
+    HRW_ERR_STATE = 4    /* object in a state that forbids the call              */
+};
+
+/* Arithmetic form.  EXACT keeps the reference's evaluation order (bit-identical to the
+ * reference Fortran compiled without FMA); FAST uses fused multiply-adds and two divides per
+ * reconstruction (agrees with EXACT to ~1e-15 relative per step). */
+enum hrw_arith { HRW_ARITH_EXACT = 0, HRW_ARITH_FAST = 1 };
+
+/* Numerical flux (src/hrweno_fluxes.f90:22-76). */
+enum hrw_numflux { HRW_FLUX_LAX_FRIEDRICHS = 0, HRW_FLUX_GODUNOV = 1 };
+
+/* Built-in physical flux functions f(u, x, t) (src/hrweno_fluxes.f90:12-18). */
+enum hrw_fluxfn {
+    HRW_FN_BURGERS = 0,  /* f = u**2/2            (example/example1_burgers_1d_fv.f90:120) */
+    HRW_FN_LINEAR = 1    /* f = (a + b*x)*u       (example/example2_pbe_2d_fv.f90:140,153) */
+};
+
+/* Boundary treatment of a problem descriptor. */
+enum hrw_bc {
+    HRW_BC_PERIODIC = 0,        /* periodic ghost cells                                   */
+    HRW_BC_ZERO_GRADIENT = 1,   /* example1:103-104: fedges(0)=fedges(1), fedges(n)=fedges(n-1) */
+    HRW_BC_ZERO_FLUX = 2,       /* example2:117-120: zero flux at both ends               */
+    HRW_BC_INFLOW0_OUTFLOW = 3  /* zero flux in at the low end, upwind f(vr(n)) out at the high end */
+};
+
+/* ---------------------------------------------------------------- runtime ---------- */
+const char *hrw_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *hrw_last_error(void);
+int hrw_device_count(int *count);
+int hrw_device_alloc(size_t bytes, void **ptr);
+int hrw_device_free(void *ptr);
+int hrw_memcpy_htod(void *dst, const void *src, size_t bytes, hrw_stream_t stream);
+int hrw_memcpy_dtoh(void *dst, const void *src, size_t bytes, hrw_stream_t stream);
+int hrw_memcpy_dtod(void *dst, const void *src, size_t bytes, hrw_stream_t stream);
+int hrw_stream_create(hrw_stream_t *stream);
+int hrw_stream_destroy(hrw_stream_t stream);
+int hrw_stream_synchronize(hrw_stream_t stream);
+/* Elapsed milliseconds between two points of a stream: record a start event, then stop. */
+int hrw_timer_start(hrw_stream_t stream);
+int hrw_timer_stop(hrw_stream_t stream, double *ms);
+
+/* ---------------------------------------------------------------- WENO ------------- */
+/* Public tables d(0:k-1) and c(0:k-1,-1:k-1) (src/hrweno_weno.f90:9-21); c is written
+ * column-major, c[(r+1)*k + j] = c(j,r).  d and c need k and k*(k+1) doubles. */
+int hrw_weno_tables(int k, double *d, double *c);
+
+/* weno(ncells, k, eps [, xedges]) -- replaces src/hrweno_weno.f90:54-127 (weno_init).
+ * xedges: host array (0:ncells) or NULL for a uniform grid; for a non-uniform grid the
+ * coefficient table cnu (src/hrweno_weno.f90:221-297) is computed on the device. */
+int hrw_weno_create(int ncells, int k, double eps, const double *xedges, int arith,
+                    hrw_weno_t *out);
+int hrw_weno_destroy(hrw_weno_t w);
+int hrw_weno_info(hrw_weno_t w, int *ncells, int *k, double *eps, int *uniform);
+/* Copy the cnu table, cnu(0:k-1,-1:k-1,1:ncells) column-major, to a host array. */
+int hrw_weno_get_cnu(hrw_weno_t w, double *cnu_host);
+
+/* w%reconstruct(v, vl, vr) -- replaces src/hrweno_weno.f90:129-219 (weno_reconstruct).
+ * Device arrays with element strides (strided sections as in example2:107). */
+int hrw_weno_reconstruct(hrw_weno_t w, const double *v, int64_t incv, double *vl,
+                         int64_t incl, double *vr, int64_t incr, hrw_stream_t stream);
+/* The same on nlines lines in one launch: element i of line l is v[l*ldv + i*incv],
+ * vl/vr[l*ldo + i*inco] (example2:97-112 does this line by line). */
+int hrw_weno_reconstruct_lines(hrw_weno_t w, int64_t nlines, const double *v, int64_t incv,
+                               int64_t ldv, double *vl, double *vr, int64_t inco,
+                               int64_t ldo, hrw_stream_t stream);
+/* Host arrays (copied through device staging buffers; synchronous). */
+int hrw_weno_reconstruct_host(hrw_weno_t w, const double *v, int64_t incv, double *vl,
+                              double *vr);
+
+/* ---------------------------------------------------------------- fluxes ----------- */
+/* h(i) = lax_friedrichs(f, vm(i), vp(i), [x(i)], t, alpha) -- src/hrweno_fluxes.f90:22-45
+ * h(i) = godunov(f, vm(i), vp(i), [x(i)], t)             -- src/hrweno_fluxes.f90:47-76
+ * for the built-in f = fn with coefficients (a, b); x may be NULL (taken as 0). */
+int hrw_flux_lax_friedrichs(int fn, double a, double b, int64_t n, const double *vm,
+                            const double *vp, const double *x, double t, double alpha,
+                            double *h, hrw_stream_t stream);
+int hrw_flux_godunov(int fn, double a, double b, int64_t n, const double *vm,
+                     const double *vp, const double *x, double t, double *h,
+                     hrw_stream_t stream);
+
+/* ---------------------------------------------------------------- grids ------------ */
+/* grid1%linear/%bilinear/%log/%geometric -- src/hrweno_grids.f90:41-230.  Host arrays:
+ * edges(0:n).  hrw_grid_compute fills center(n), width(n) (src/hrweno_grids.f90:232-250). */
+int hrw_grid_linear(double xmin, double xmax, int ncells, double *edges);
+int hrw_grid_bilinear(double xmin, double xcross, double xmax, int ncells1, int ncells2,
+                      double *edges);
+int hrw_grid_log(double xmin, double xmax, int ncells, double *edges);
+int hrw_grid_geometric(double xmin, double xmax, double ratio, int ncells, double *edges);
+int hrw_grid_compute(int ncells, const double *edges, double *center, double *width);
+
+/* ---------------------------------------------------------------- problems --------- */
+/* A device problem descriptor stands for the user RHS `fu` of the reference examples, so
+ * that the integrators can fuse RHS and stage update into one pass per stage.
+ *
+ * 1D Burgers u_t + (u**2/2)_x = 0 on a linear grid [xmin, xmax] of n cells, or on the cell
+ * widths `width` (host, n; NULL = linear grid) -- example/example1_burgers_1d_fv.f90:72-122.
+ * alpha < 0 selects alpha = max|v| of each stage input for Lax-Friedrichs. */
+int hrw_problem_burgers1d(int n, int k, double eps, int numflux, int bc, double alpha,
+                          double xmin, double xmax, const double *width, int arith,
+                          hrw_problem_t *out);
+/* 2D population balance n_t + (g1 n)_x1 + (g2 n)_x2 = 0, g1 = g1a + g1b*x1,
+ * g2 = g2a + g2b*x2, Godunov flux, linear grids, u((j-1)*n1 + i) --
+ * example/example2_pbe_2d_fv.f90:73-155.  bc: HRW_BC_ZERO_FLUX or HRW_BC_INFLOW0_OUTFLOW. */
+int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, double x1max,
+                      double x2min, double x2max, double g1a, double g1b, double g2a,
+                      double g2b, int arith, hrw_problem_t *out);
+int hrw_problem_destroy(hrw_problem_t p);
+int hrw_problem_neq(hrw_problem_t p, int64_t *neq);
+/* One RHS evaluation udot = L(u) on device arrays (the user `rhs` of the examples). */
+int hrw_problem_rhs(hrw_problem_t p, double t, const double *u, double *udot,
+                    hrw_stream_t stream);
+
+/* ---------------------------------------------------------------- integrators ------ */
+/* Callback integrand on device arrays: udot = f(t, u); must enqueue its work on `stream`
+ * and return 0 on success (src/hrweno_tvdode.f90:50-57, abstract interface integrand). */
+typedef int (*hrw_integrand_fn)(void *ctx, double t, const double *u, double *udot,
+                                hrw_stream_t stream);
+
+/* rktvd(fu, neq, order) -- src/hrweno_tvdode.f90:69-95 (fused device problem or callback) */
+int hrw_rktvd_create(hrw_problem_t p, int order, hrw_ode_t *out);
+int hrw_rktvd_create_cb(hrw_integrand_fn fu, void *ctx, int64_t neq, int order, int arith,
+                        hrw_ode_t *out);
+/* mstvd(fu, neq) -- src/hrweno_tvdode.f90:180-201 */
+int hrw_mstvd_create(hrw_problem_t p, hrw_ode_t *out);
+int hrw_mstvd_create_cb(hrw_integrand_fn fu, void *ctx, int64_t neq, int arith,
+                        hrw_ode_t *out);
+int hrw_ode_destroy(hrw_ode_t ode);
+int hrw_ode_set_stream(hrw_ode_t ode, hrw_stream_t stream);
+/* ode%integrate(u, t, tout, dt [, itask]) -- src/hrweno_tvdode.f90:97-178 (rktvd) and
+ * :203-271 (mstvd; itask ignored).  u is a device array of neq doubles; t is updated with
+ * the reference's loop control (is_done, :273-284), including its overshoot of tout. */
+int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tout, double dt, int itask);
+/* Public fields neq, order, fevals, istate (src/hrweno_tvdode.f90:18-29). */
+int hrw_ode_info(hrw_ode_t ode, int64_t *neq, int *order, int *fevals, int *istate);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HRWENO_CAPI_H */

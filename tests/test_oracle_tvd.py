@@ -1,0 +1,217 @@
+"""Pins the oracle's fluxes (src/hrweno_fluxes.f90), grids (src/hrweno_grids.f90) and TVD
+integrators (src/hrweno_tvdode.f90), plus the Burgers/PBE right-hand sides.
+
+Closed forms: one RK step on u' = a*u multiplies by R(z), z = a*dt, with
+R1 = 1+z, R2 = 1+z+z^2/2, R3 = 1+z+z^2/2+z^3/6; the 5-step multistep method follows
+u_{n+1} = ((25+50z) u_n + (7+10z) u_{n-4}) / 32 after four RK3 steps.
+"""
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+A = -1.0 + np.arange(10) * 4.0 / 9          # test/test_tvdode.f90:15
+
+
+# ------------------------------------------------------------------------------ fluxes
+def test_reference_fluxes_known_answers(oracle):
+    """test/test_fluxes.f90:27-71 with f = u*x*t, x = 3, t = 5 (alpha = x*t)."""
+    f = lambda u: u * 3.0 * 5.0  # noqa: E731
+    for vm, vp, href in [(2.0, 2.0, 30.0), (2.0, -4.0, 30.0), (-2.0, 4.0, -30.0)]:
+        assert abs(oracle.godunov(f(vm), f(vp), vm, vp) - href) <= 1e-8 * abs(href)
+        assert abs(oracle.lax_friedrichs(f(vm), f(vp), vm, vp, 15.0) - href) <= 1e-8 * abs(href)
+
+
+def test_fluxes_vs_reference_build(oracle, ref):
+    rng = np.random.default_rng(5)
+    vm, vp = rng.standard_normal(500), rng.standard_normal(500)
+    x = rng.uniform(0, 2, 500)
+    vm[:50] = vp[:50]
+    for which in (0, 1):
+        h = ref.fluxes(0, which, vm, vp, x, 0.7, alpha=1.3)
+        for i in range(500):
+            fm, fp = vm[i] ** 2 / 2, vp[i] ** 2 / 2
+            mine = (oracle.lax_friedrichs(fm, fp, vm[i], vp[i], 1.3) if which == 0
+                    else oracle.godunov(fm, fp, vm[i], vp[i]))
+            assert mine == h[i]
+
+
+def test_godunov_is_upwind_for_linear_flux(oracle):
+    rng = np.random.default_rng(6)
+    for _ in range(200):
+        g = rng.uniform(0.1, 3)
+        vm, vp = rng.standard_normal(2)
+        assert oracle.godunov(g * vm, g * vp, vm, vp) == g * vm
+        assert oracle.godunov(-g * vm, -g * vp, vm, vp) == -g * vp
+
+
+# ------------------------------------------------------------------------------ grids
+def _check_grid(e, xmin, xmax, n, oracle):
+    c, w = oracle.grid_compute(e)
+    assert e.size == n + 1
+    assert abs(e[0] - xmin) <= 1e-5 * abs(xmin) + (0 if xmin else 0)
+    assert abs(e[-1] - xmax) <= 1e-5 * abs(xmax)
+    assert np.array_equal(w, e[1:] - e[:-1])
+    assert np.array_equal(c, (e[:-1] + e[1:]) / 2)
+
+
+def test_reference_grids(oracle):
+    """test/test_grid.f90:31-162 (inputs and tolerances of the reference)."""
+    _check_grid(oracle.grid_linear(1.0, 1e3, 10 ** 6), 1.0, 1e3, 10 ** 6, oracle)
+    _check_grid(oracle.grid_log(1e-1, 1e3, 10 ** 4), 1e-1, 1e3, 10 ** 4, oracle)
+    _check_grid(oracle.grid_geometric(1e1, 1e3, 1.1, 10 ** 2), 1e1, 1e3, 10 ** 2, oracle)
+    e = oracle.grid_bilinear(0.0, 1e1, 1e3, 124, 365)
+    assert e.size == 490 and e[0] == 0.0 and abs(e[124] - 10.0) <= 1e-4
+    assert abs(e[-1] - 1e3) <= 1e-2
+
+
+def test_grid_formulas(oracle):
+    n = 1000
+    e = oracle.grid_linear(-1.0, 1.0, n)
+    rx = 2.0 / n
+    assert np.array_equal(e, -1.0 + rx * np.arange(n + 1))
+    g = oracle.grid_geometric(1.0, 2.0, 1.5, 7)
+    w = np.diff(g)
+    assert np.allclose(w[1:] / w[:-1], 1.5, rtol=1e-12)
+    lg = oracle.grid_log(0.5, 50.0, 20)
+    assert np.allclose(np.diff(np.log(lg)), np.log(100.0) / 20, rtol=1e-12)
+    for bad in [lambda: oracle.grid_linear(1.0, 1.0, 3), lambda: oracle.grid_log(0.0, 1.0, 3),
+                lambda: oracle.grid_geometric(0.0, 1.0, -1.0, 3)]:
+        with pytest.raises(ValueError):
+            bad()
+
+
+# ------------------------------------------------------------------------------ TVD ODEs
+def _steps_taken(t0, tout, dt):
+    t, n = t0, 0
+    while True:
+        t = t + dt
+        n += 1
+        if (t - tout) * np.copysign(1.0, dt) > 0:
+            return n, t
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_rktvd_closed_form(oracle, order):
+    t0, tout = -1.3, 1.5
+    dt = (tout - t0) / 3000
+    u, t, fe = oracle.linear_ode(A, np.full(10, 0.1), order, t0, tout, dt)
+    nsteps, tref = _steps_taken(t0, tout, dt)
+    assert t == tref and fe == order * nsteps
+    z = A * dt
+    R = {1: 1 + z, 2: 1 + z + z ** 2 / 2, 3: 1 + z + z ** 2 / 2 + z ** 3 / 6}[order]
+    assert np.allclose(u, 0.1 * R ** nsteps, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_reference_rktvd(oracle, order):
+    """test/test_tvdode.f90:31-70: compare with u0*exp(a(t-t0)) at the returned t."""
+    t0, tout = -1.3, 1.5
+    u, t, _ = oracle.linear_ode(A, np.full(10, 0.1), order, t0, tout, (tout - t0) / 3000)
+    rtol = [2e-2, 1e-3, 1e-3][order - 1]
+    assert np.allclose(u, 0.1 * np.exp(A * (t - t0)), rtol=rtol, atol=0)
+
+
+def _mstvd_rational(a, dt, nsteps_main):
+    """Exact-rational startup (4 RK3 steps) + multistep recurrence for u' = a u."""
+    z = Fr(a) * Fr(dt)
+    R3 = 1 + z + z * z / 2 + z * z * z / 6
+    hist = [Fr(1, 10)]
+    for _ in range(4):
+        hist.append(hist[-1] * R3)
+    for _ in range(nsteps_main):
+        hist.append(((25 + 50 * z) * hist[-1] + (7 + 10 * z) * hist[-5]) / 32)
+    return hist[-1]
+
+
+def test_mstvd_exact_recurrence(oracle):
+    t0, dt = 0.0, 1.0 / 64
+    tout = 20 * dt + dt / 2
+    u, t, fe = oracle.linear_ode(A, np.full(10, 0.1), 0, t0, tout, dt)
+    nmain = fe - 12
+    assert nmain == 17     # 4 startup steps (not counted per step) + 17 main steps
+    for i, a in enumerate(A):
+        exact = float(_mstvd_rational(a, dt, nmain))
+        assert abs(u[i] - exact) <= 1e-13 * abs(exact)
+
+
+def test_reference_mstvd(oracle):
+    """test/test_tvdode.f90:72-104."""
+    t0, tout = -1.3, 1.5
+    u, t, fe = oracle.linear_ode(A, np.full(10, 0.1), 0, t0, tout, 1e-3)
+    assert np.allclose(u, 0.1 * np.exp(A * (t - t0)), rtol=1e-3, atol=0)
+
+
+def test_mstvd_startup_ignores_tout(oracle):
+    """Four RK3 starting steps are taken even when tout is reached earlier."""
+    u, t, fe = oracle.linear_ode(A, np.full(10, 0.1), 0, 0.0, 0.01, 0.1)
+    assert fe == 12 and abs(t - 0.4) < 1e-15
+
+
+# ------------------------------------------------------------------------------ Burgers
+def _burgers_setup(n, xmin=-1.0, xmax=1.0):
+    from hrweno_hip import workloads as W
+    return W.linear_centres(xmin, xmax, n)
+
+
+def test_burgers_periodic_conserves_mass(oracle):
+    e, xc, w = _burgers_setup(200)
+    u0 = 0.5 + np.sin(np.pi * xc) + (xc > 0.1)
+    u, t, steps, fe = oracle.burgers_rk(u0, w, tout=0.3, dt=0.4 * 0.01 / 2.0)
+    assert steps > 10 and fe == 3 * steps
+    assert abs(np.sum(u * w) - np.sum(u0 * w)) <= 1e-13 * np.sum(np.abs(u0) * w)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_burgers_constant_state_is_fixed(oracle, flux):
+    e, xc, w = _burgers_setup(64)
+    u0 = np.full(64, 0.75)
+    u, *_ = oracle.burgers_rk(u0, w, flux=flux, tout=0.1, dt=0.01)
+    assert np.array_equal(u, u0)
+
+
+def test_burgers_shock_speed(oracle):
+    """Riemann problem uL=1, uR=0: the shock moves at (uL+uR)/2 (Rankine-Hugoniot)."""
+    n = 400
+    e, xc, w = _burgers_setup(n, 0.0, 1.0)
+    u0 = np.where(xc < 0.25, 1.0, 0.0)
+    u, t, *_ = oracle.burgers_rk(u0, w, flux=1, bc=1, tout=0.4, dt=0.4 / n)
+    pos = 0.0 + np.sum(u * w)    # mass = shock position for uL = 1 (zero-gradient ends)
+    assert abs(pos - (0.25 + 0.5 * t)) < 2.0 / n
+
+
+def test_periodic_padding_equals_wrap(oracle):
+    """The periodic RHS (padding with k ghost cells) equals a wrapped-index restatement."""
+    n, k = 37, 3
+    rng = np.random.default_rng(8)
+    u0 = rng.standard_normal(n)
+    e, xc, w = _burgers_setup(n)
+    u1, *_ = oracle.burgers_rk(u0, w, flux=1, order=1, tout=1e-4, dt=1e-3, itask=2)
+    vr = np.empty(n)
+    vl = np.empty(n)
+    for i in range(n):
+        win = np.array([u0[(i + m) % n] for m in range(-(k - 1), k)])
+        a, b = oracle.reconstruct(win, k)
+        vl[i], vr[i] = a[k - 1], b[k - 1]
+    F = np.array([oracle.godunov(vr[i] ** 2 / 2, vl[(i + 1) % n] ** 2 / 2, vr[i], vl[(i + 1) % n])
+                  for i in range(n)])
+    L = -(F - np.roll(F, 1)) / w
+    assert np.array_equal(u1, u0 + 1e-3 * L)
+
+
+def test_pbe_mass_balance(oracle):
+    """Zero-flux boundaries (example 2): total mass is conserved by the 2D scheme."""
+    n1, n2 = 24, 20
+    e1 = np.linspace(0, 1, n1 + 1)
+    e2 = np.linspace(0, 1, n2 + 1)
+    from hrweno_hip import workloads as W
+    _, c1, w1 = W.linear_centres(0, 1, n1)
+    _, c2, w2 = W.linear_centres(0, 1, n2)
+    e1, _, _ = W.linear_centres(0, 1, n1)
+    e2, _, _ = W.linear_centres(0, 1, n2)
+    X1, X2 = np.meshgrid(c1, c2, indexing="xy")
+    u0 = np.exp(-((X1 - 0.4) ** 2 + (X2 - 0.4) ** 2) / 0.02).ravel()
+    u, t, fe = oracle.pbe_ms(u0, e1, e2, bc=0, g1=(1.0, 0.5), g2=(0.5, 0.0), tout=0.05,
+                             dt=0.002)
+    mass = lambda q: np.sum(q.reshape(n2, n1) * w1[None, :] * w2[:, None])  # noqa: E731
+    assert fe >= 12 and abs(mass(u) - mass(u0)) <= 1e-13 * mass(u0)
