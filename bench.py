@@ -54,13 +54,26 @@ def cpu_baseline(seconds: float):
                                          dt=c.dt)
         kind = "port"
         what = "oracle C restatement (gcc -O3, no FMA)"
-    t0 = time.perf_counter()
-    run(2)
-    per_step = (time.perf_counter() - t0) / 2
-    steps = max(2, int(seconds / per_step))
-    t0 = time.perf_counter()
-    run(steps)
-    dt = time.perf_counter() - t0
+    # The reference keeps O(n) automatic arrays on the stack (src/hrweno_weno.f90:163), so the
+    # timed calls run on a thread with a 1 GiB stack.
+    import threading
+    res = {}
+
+    def worker():
+        t0 = time.perf_counter()
+        run(2)
+        per_step = (time.perf_counter() - t0) / 2
+        res["steps"] = steps = max(2, int(seconds / per_step))
+        t0 = time.perf_counter()
+        run(steps)
+        res["time"] = time.perf_counter() - t0
+
+    old_size = threading.stack_size(1 << 30)
+    th = threading.Thread(target=worker)
+    th.start()
+    th.join()
+    threading.stack_size(old_size)
+    steps, dt = res["steps"], res["time"]
     return {"value": n * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": kind,
             "sample": f"{n} cells x {steps} RK3 steps of the same workload (WENO5-LF, periodic), "
                       f"{dt:.1f} s on one host core; {what}"}
@@ -75,6 +88,9 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of --cells doubles (20 times), the "
+                         "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards "
@@ -86,6 +102,14 @@ def main() -> int:
     from hrweno_hip import workloads as W
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.rand(args.cells, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 20 copies of {args.cells * 8} bytes")
+        return 0
     log(f"generating workload: {args.cells} cells")
     c = W.config2(n=args.cells)
     ode = hh.RkTvd(hh.Burgers1D(c.n, 3, 1e-6, flux=hh.LAX_FRIEDRICHS, bc=hh.BC_PERIODIC,

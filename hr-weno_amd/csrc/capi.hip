@@ -236,13 +236,7 @@ struct hrw_weno_s {
     double eps = 1e-6;
     bool uniform = true;
     double *d_cnu = nullptr;                 // (n, k+1, k) on the device
-    double *stage[3] = {nullptr, nullptr, nullptr};   // host-array staging
-    int64_t stage_n = 0;
-    ~hrw_weno_s()
-    {
-        dfree(d_cnu);
-        for (auto &p : stage) dfree(p);
-    }
+    ~hrw_weno_s() { dfree(d_cnu); }
 };
 
 extern "C" int hrw_weno_tables(int k, double *d, double *c)
@@ -294,18 +288,101 @@ __global__ void calc_cnu_kernel(int n, int k, const double *xext, double *cnu)
     cnu[t] = sum2 * (XR(i - r + j) - XL(i - r + j));
 }
 
-extern "C" int hrw_weno_create(int ncells, int k, double eps, const double *xedges, int arith,
-                               hrw_weno_t *out)
+// Thread-local device staging for host-array calls (grows, never shrinks).
+struct Staging {
+    double *buf = nullptr;
+    size_t cap = 0;
+    ~Staging() { dfree(buf); }
+    int reserve(size_t count)
+    {
+        if (count <= cap) return HRW_OK;
+        dfree(buf);
+        cap = 0;
+        CALL(dalloc(&buf, count));
+        cap = count;
+        return HRW_OK;
+    }
+};
+static thread_local Staging g_stage;
+
+static int upload_strided(double *dst, const double *src, int64_t n, int64_t inc)
 {
-    if (!out) return fail(HRW_ERR_ARG, "hrw_weno_create: null output pointer");
-    *out = nullptr;
-    // src/hrweno_weno.f90:72-98
-    if (!(ncells > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
+    if (inc == 1) {
+        HIP_TRY(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy2D(dst, sizeof(double), src, inc * sizeof(double), sizeof(double), n,
+                            hipMemcpyHostToDevice));
+    }
+    return HRW_OK;
+}
+static int download_strided(double *dst, int64_t inc, const double *src, int64_t n)
+{
+    if (inc == 1) {
+        HIP_TRY(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        HIP_TRY(hipMemcpy2D(dst, inc * sizeof(double), src, sizeof(double), sizeof(double), n,
+                            hipMemcpyDeviceToHost));
+    }
+    return HRW_OK;
+}
+
+static int check_weno_args(int n, int k, double eps, int arith)
+{
+    if (!(n > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
     if (!(k >= 1 && k <= 3)) return fail(HRW_ERR_ARG, "Invalid input 'k'. Valid range: 1 <= k <= 3.");
     if (!(eps > 2.220446049250313e-16))
         return fail(HRW_ERR_ARG, "Invalid input 'eps'. Valid range: eps > epsilon.");
     if (arith != HRW_ARITH_EXACT && arith != HRW_ARITH_FAST)
         return fail(HRW_ERR_ARG, "Invalid arithmetic form %d", arith);
+    return HRW_OK;
+}
+
+// cnu on the device from host edges xedges(0:n) (src/hrweno_weno.f90:241-255 extend the
+// grid by k+1 ghost edges on each side; the extension is a handful of host subtractions).
+static int compute_cnu(int n, int k, const double *xedges, double *d_cnu)
+{
+    const int ng = k + 1;
+    std::vector<double> xext(n + 1 + 2 * ng);
+    for (int i = 0; i <= n; ++i) xext[i + ng] = xedges[i];
+    double dx = xext[1 + ng] - xext[0 + ng];               // :246-249
+    for (int i = -1; i >= -ng; --i) xext[i + ng] = xext[i + 1 + ng] - dx;
+    dx = xext[n + ng] - xext[n - 1 + ng];                  // :252-255
+    for (int i = n + 1; i <= n + ng; ++i) xext[i + ng] = xext[i - 1 + ng] + dx;
+    double *d_xext = nullptr;
+    CALL(dalloc(&d_xext, xext.size()));
+    hipError_t e = hipMemcpy(d_xext, xext.data(), xext.size() * sizeof(double),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const long long tot = (long long)n * k * (k + 1);
+        calc_cnu_kernel<<<grid_for(tot, 256, 1u << 30), 256>>>(n, k, d_xext, d_cnu);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    dfree(d_xext);
+    if (e != hipSuccess) return fail(HRW_ERR_HIP, "cnu setup failed: %s", hipGetErrorString(e));
+    return HRW_OK;
+}
+
+extern "C" int hrw_weno_calc_cnu(int ncells, int k, const double *xedges, double *cnu_host)
+{
+    CALL(check_weno_args(ncells, k, 1.0, HRW_ARITH_EXACT));
+    if (!xedges || !cnu_host) return fail(HRW_ERR_ARG, "null array");
+    if (ncells < 2) return fail(HRW_ERR_ARG, "Invalid input 'xedges': a non-uniform grid needs ncells >= 2.");
+    const size_t count = (size_t)ncells * k * (k + 1);
+    CALL(g_stage.reserve(count));
+    CALL(compute_cnu(ncells, k, xedges, g_stage.buf));
+    HIP_TRY(hipMemcpy(cnu_host, g_stage.buf, count * sizeof(double), hipMemcpyDeviceToHost));
+    return HRW_OK;
+}
+
+extern "C" int hrw_weno_create(int ncells, int k, double eps, const double *xedges, int arith,
+                               hrw_weno_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "hrw_weno_create: null output pointer");
+    *out = nullptr;
+    CALL(check_weno_args(ncells, k, eps, arith));          // src/hrweno_weno.f90:72-98
+    if (xedges && ncells < 2)
+        return fail(HRW_ERR_ARG, "Invalid input 'xedges': a non-uniform grid needs ncells >= 2.");
     auto *w = new (std::nothrow) hrw_weno_s;
     if (!w) return fail(HRW_ERR_ALLOC, "out of host memory");
     w->n = ncells;
@@ -314,33 +391,11 @@ extern "C" int hrw_weno_create(int ncells, int k, double eps, const double *xedg
     w->arith = arith;
     if (xedges) {   // :100-109
         w->uniform = false;
-        const int ng = k + 1;
-        std::vector<double> xext(ncells + 1 + 2 * ng);
-        for (int i = 0; i <= ncells; ++i) xext[i + ng] = xedges[i];
-        double dx = xext[1 + ng] - xext[0 + ng];               // :246-249
-        for (int i = -1; i >= -ng; --i) xext[i + ng] = xext[i + 1 + ng] - dx;
-        dx = xext[ncells + ng] - xext[ncells - 1 + ng];        // :252-255
-        for (int i = ncells + 1; i <= ncells + ng; ++i) xext[i + ng] = xext[i - 1 + ng] + dx;
-        double *d_xext = nullptr;
-        int rc = dalloc(&d_xext, xext.size());
-        if (rc == HRW_OK) rc = dalloc(&w->d_cnu, (size_t)ncells * k * (k + 1));
+        int rc = dalloc(&w->d_cnu, (size_t)ncells * k * (k + 1));
+        if (rc == HRW_OK) rc = compute_cnu(ncells, k, xedges, w->d_cnu);
         if (rc != HRW_OK) {
-            dfree(d_xext);
             delete w;
             return rc;
-        }
-        hipError_t e = hipMemcpy(d_xext, xext.data(), xext.size() * sizeof(double),
-                                 hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            const long long tot = (long long)ncells * k * (k + 1);
-            calc_cnu_kernel<<<grid_for(tot, 256, 1u << 30), 256>>>(ncells, k, d_xext, w->d_cnu);
-            e = hipGetLastError();
-            if (e == hipSuccess) e = hipDeviceSynchronize();
-        }
-        dfree(d_xext);
-        if (e != hipSuccess) {
-            delete w;
-            return fail(HRW_ERR_HIP, "cnu setup failed: %s", hipGetErrorString(e));
         }
     }
     *out = w;
@@ -372,21 +427,45 @@ extern "C" int hrw_weno_get_cnu(hrw_weno_t w, double *cnu_host)
     return HRW_OK;
 }
 
-template <int K>
-static void launch_reconstruct(const hrw_weno_s *w, int64_t nlines, const double *v, int64_t incv,
-                               int64_t ldv, double *vl, double *vr, int64_t inco, int64_t ldo,
-                               hipStream_t s)
+// Reconstruction of nlines lines on the device; cnu (device, or nullptr) selects the
+// non-uniform coefficients.
+struct ReconArgs {
+    long long n;
+    int k, arith;
+    double eps;
+    const double *cnu;
+    long long nlines;
+    const double *v;
+    long long incv, ldv;
+    double *vl;
+    long long incl, ldl;
+    double *vr;
+    long long incr, ldr;
+};
+
+template <int K> static void launch_reconstruct(const ReconArgs &a, hipStream_t s)
 {
-    dim3 grid(grid_for(w->n, kBlock, 1u << 30), (unsigned)nlines);
-    if (!w->uniform)
-        reconstruct_kernel<K, true, false><<<grid, kBlock, 0, s>>>(v, incv, ldv, w->n, w->eps,
-                                                                  w->d_cnu, vl, vr, inco, ldo);
-    else if (w->arith == HRW_ARITH_FAST)
-        reconstruct_kernel<K, false, true><<<grid, kBlock, 0, s>>>(v, incv, ldv, w->n, w->eps,
-                                                                  nullptr, vl, vr, inco, ldo);
+    dim3 grid(grid_for(a.n, kBlock, 1u << 30), (unsigned)a.nlines);
+    if (a.cnu)
+        reconstruct_kernel<K, true, false><<<grid, kBlock, 0, s>>>(
+            a.v, a.incv, a.ldv, a.n, a.eps, a.cnu, a.vl, a.incl, a.ldl, a.vr, a.incr, a.ldr);
+    else if (a.arith == HRW_ARITH_FAST)
+        reconstruct_kernel<K, false, true><<<grid, kBlock, 0, s>>>(
+            a.v, a.incv, a.ldv, a.n, a.eps, nullptr, a.vl, a.incl, a.ldl, a.vr, a.incr, a.ldr);
     else
-        reconstruct_kernel<K, false, false><<<grid, kBlock, 0, s>>>(v, incv, ldv, w->n, w->eps,
-                                                                   nullptr, vl, vr, inco, ldo);
+        reconstruct_kernel<K, false, false><<<grid, kBlock, 0, s>>>(
+            a.v, a.incv, a.ldv, a.n, a.eps, nullptr, a.vl, a.incl, a.ldl, a.vr, a.incr, a.ldr);
+}
+
+static int reconstruct_core(const ReconArgs &a, hipStream_t s)
+{
+    if (a.nlines < 1 || a.nlines > 65535) return fail(HRW_ERR_ARG, "nlines must be in [1, 65535]");
+    if (!a.v || !a.vl || !a.vr) return fail(HRW_ERR_ARG, "null array");
+    if (a.k == 1) launch_reconstruct<1>(a, s);
+    else if (a.k == 2) launch_reconstruct<2>(a, s);
+    else launch_reconstruct<3>(a, s);
+    LAUNCH_CHECK();
+    return HRW_OK;
 }
 
 extern "C" int hrw_weno_reconstruct_lines(hrw_weno_t w, int64_t nlines, const double *v,
@@ -394,47 +473,57 @@ extern "C" int hrw_weno_reconstruct_lines(hrw_weno_t w, int64_t nlines, const do
                                           int64_t inco, int64_t ldo, hrw_stream_t stream)
 {
     if (!w) return fail(HRW_ERR_ARG, "null weno object");
-    if (nlines < 1 || nlines > 65535) return fail(HRW_ERR_ARG, "nlines must be in [1, 65535]");
-    if (!v || !vl || !vr) return fail(HRW_ERR_ARG, "null array");
-    if (w->k == 1) launch_reconstruct<1>(w, nlines, v, incv, ldv, vl, vr, inco, ldo, S(stream));
-    else if (w->k == 2) launch_reconstruct<2>(w, nlines, v, incv, ldv, vl, vr, inco, ldo, S(stream));
-    else launch_reconstruct<3>(w, nlines, v, incv, ldv, vl, vr, inco, ldo, S(stream));
-    LAUNCH_CHECK();
-    return HRW_OK;
+    ReconArgs a{w->n, w->k, w->arith, w->eps, w->d_cnu, nlines, v, incv, ldv,
+                vl, inco, ldo, vr, inco, ldo};
+    return reconstruct_core(a, S(stream));
 }
 
 extern "C" int hrw_weno_reconstruct(hrw_weno_t w, const double *v, int64_t incv, double *vl,
                                     int64_t incl, double *vr, int64_t incr, hrw_stream_t stream)
 {
     if (!w) return fail(HRW_ERR_ARG, "null weno object");
-    if (incl != incr) {
-        // different output strides: reconstruct into vl/vr via two launches of the same kernel
-        // would double the work; require equal strides (contiguous outputs are the norm).
-        return fail(HRW_ERR_ARG, "vl and vr must have the same stride");
-    }
-    return hrw_weno_reconstruct_lines(w, 1, v, incv, 0, vl, vr, incl, 0, stream);
+    ReconArgs a{w->n, w->k, w->arith, w->eps, w->d_cnu, 1, v, incv, 0, vl, incl, 0, vr, incr, 0};
+    return reconstruct_core(a, S(stream));
+}
+
+extern "C" int hrw_reconstruct_host(int n, int k, double eps, int arith, const double *cnu,
+                                    const double *v, int64_t incv, double *vl, int64_t incl,
+                                    double *vr, int64_t incr)
+{
+    CALL(check_weno_args(n, k, eps, arith));
+    const size_t ncnu = cnu ? (size_t)n * k * (k + 1) : 0;
+    CALL(g_stage.reserve(3 * (size_t)n + ncnu));
+    double *dv = g_stage.buf, *dl = dv + n, *dr = dl + n, *dc = cnu ? dr + n : nullptr;
+    CALL(upload_strided(dv, v, n, incv));
+    if (cnu) HIP_TRY(hipMemcpy(dc, cnu, ncnu * sizeof(double), hipMemcpyHostToDevice));
+    ReconArgs a{n, k, arith, eps, dc, 1, dv, 1, 0, dl, 1, 0, dr, 1, 0};
+    CALL(reconstruct_core(a, nullptr));
+    CALL(download_strided(vl, incl, dl, n));
+    CALL(download_strided(vr, incr, dr, n));
+    return HRW_OK;
+}
+
+extern "C" int hrw_reconstruct_dev(int n, int k, double eps, int arith, const double *v,
+                                   int64_t incv, double *vl, int64_t incl, double *vr,
+                                   int64_t incr, hrw_stream_t stream)
+{
+    CALL(check_weno_args(n, k, eps, arith));
+    ReconArgs a{n, k, arith, eps, nullptr, 1, v, incv, 0, vl, incl, 0, vr, incr, 0};
+    return reconstruct_core(a, S(stream));
 }
 
 extern "C" int hrw_weno_reconstruct_host(hrw_weno_t w, const double *v, int64_t incv,
                                          double *vl, double *vr)
 {
     if (!w) return fail(HRW_ERR_ARG, "null weno object");
+    CALL(g_stage.reserve(3 * (size_t)w->n));
     const int64_t n = w->n;
-    if (w->stage_n != n) {
-        for (auto &p : w->stage) dfree(p);
-        for (auto &p : w->stage) CALL(dalloc(&p, n));
-        w->stage_n = n;
-    }
-    if (incv == 1) {
-        HIP_TRY(hipMemcpy(w->stage[0], v, n * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-        HIP_TRY(hipMemcpy2D(w->stage[0], sizeof(double), v, incv * sizeof(double), sizeof(double),
-                            n, hipMemcpyHostToDevice));
-    }
-    CALL(hrw_weno_reconstruct_lines(w, 1, w->stage[0], 1, 0, w->stage[1], w->stage[2], 1, 0,
-                                    nullptr));
-    HIP_TRY(hipMemcpy(vl, w->stage[1], n * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(vr, w->stage[2], n * sizeof(double), hipMemcpyDeviceToHost));
+    double *dv = g_stage.buf, *dl = dv + n, *dr = dl + n;
+    CALL(upload_strided(dv, v, n, incv));
+    ReconArgs a{w->n, w->k, w->arith, w->eps, w->d_cnu, 1, dv, 1, 0, dl, 1, 0, dr, 1, 0};
+    CALL(reconstruct_core(a, nullptr));
+    CALL(download_strided(vl, 1, dl, n));
+    CALL(download_strided(vr, 1, dr, n));
     return HRW_OK;
 }
 

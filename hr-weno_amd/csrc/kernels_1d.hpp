@@ -98,8 +98,8 @@ __global__ __launch_bounds__(kBlock) void maxabs_kernel(const double *__restrict
 template <int K, bool NONUNIFORM, bool FAST>
 __global__ __launch_bounds__(kBlock) void reconstruct_kernel(
     const double *__restrict__ v, long long incv, long long ldv, long long n, double eps,
-    const double *__restrict__ cnu, double *__restrict__ vl, double *__restrict__ vr,
-    long long inco, long long ldo)
+    const double *__restrict__ cnu, double *__restrict__ vl, long long incl, long long ldl,
+    double *__restrict__ vr, long long incr, long long ldr)
 {
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -116,8 +116,8 @@ __global__ __launch_bounds__(kBlock) void reconstruct_kernel(
     } else {
         weno_cell_exact<K>(x, eps, nullptr, a, b);
     }
-    vl[l * ldo + i * inco] = a;
-    vr[l * ldo + i * inco] = b;
+    vl[l * ldl + i * incl] = a;
+    vr[l * ldr + i * incr] = b;
 }
 
 // --------------------------------------------------------------------------------------

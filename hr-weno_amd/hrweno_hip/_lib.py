@@ -44,6 +44,9 @@ SIGNATURES = {
     "hrw_weno_reconstruct": [P, P, L, P, L, P, L, P],
     "hrw_weno_reconstruct_lines": [P, L, P, L, L, P, P, L, L, P],
     "hrw_weno_reconstruct_host": [P, PD, L, PD, PD],
+    "hrw_weno_calc_cnu": [I, I, PD, PD],
+    "hrw_reconstruct_host": [I, I, D, I, PD, PD, L, PD, L, PD, L],
+    "hrw_reconstruct_dev": [I, I, D, I, P, L, P, L, P, L, P],
     "hrw_flux_lax_friedrichs": [I, D, D, L, P, P, P, D, D, P, P],
     "hrw_flux_godunov": [I, D, D, L, P, P, P, D, P, P],
     "hrw_grid_linear": [D, D, I, PD],

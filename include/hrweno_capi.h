@@ -86,6 +86,9 @@ int hrw_weno_destroy(hrw_weno_t w);
 int hrw_weno_info(hrw_weno_t w, int *ncells, int *k, double *eps, int *uniform);
 /* Copy the cnu table, cnu(0:k-1,-1:k-1,1:ncells) column-major, to a host array. */
 int hrw_weno_get_cnu(hrw_weno_t w, double *cnu_host);
+/* Stateless form of the same table -- replaces src/hrweno_weno.f90:221-297 (weno_calc_cnu):
+ * host xedges(0:ncells) in, host cnu out, computed on the device. */
+int hrw_weno_calc_cnu(int ncells, int k, const double *xedges, double *cnu_host);
 
 /* w%reconstruct(v, vl, vr) -- replaces src/hrweno_weno.f90:129-219 (weno_reconstruct).
  * Device arrays with element strides (strided sections as in example2:107). */
@@ -99,6 +102,15 @@ int hrw_weno_reconstruct_lines(hrw_weno_t w, int64_t nlines, const double *v, in
 /* Host arrays (copied through device staging buffers; synchronous). */
 int hrw_weno_reconstruct_host(hrw_weno_t w, const double *v, int64_t incv, double *vl,
                               double *vr);
+/* Stateless forms for hosts that keep the weno object themselves (the Fortran module):
+ * the same src/hrweno_weno.f90:129-219 on host arrays (cnu: host table or NULL) or on
+ * device arrays (uniform grid). */
+int hrw_reconstruct_host(int ncells, int k, double eps, int arith, const double *cnu,
+                         const double *v, int64_t incv, double *vl, int64_t incl, double *vr,
+                         int64_t incr);
+int hrw_reconstruct_dev(int ncells, int k, double eps, int arith, const double *v,
+                        int64_t incv, double *vl, int64_t incl, double *vr, int64_t incr,
+                        hrw_stream_t stream);
 
 /* ---------------------------------------------------------------- fluxes ----------- */
 /* h(i) = lax_friedrichs(f, vm(i), vp(i), [x(i)], t, alpha) -- src/hrweno_fluxes.f90:22-45
