@@ -87,6 +87,9 @@ def main() -> int:
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cells", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="1D FAST kernel: 0 register-blocked, 1 LDS tile")
+    ap.add_argument("--cells-per-lane", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of --cells doubles (20 times), the "
@@ -112,8 +115,10 @@ def main() -> int:
         return 0
     log(f"generating workload: {args.cells} cells")
     c = W.config2(n=args.cells)
-    ode = hh.RkTvd(hh.Burgers1D(c.n, 3, 1e-6, flux=hh.LAX_FRIEDRICHS, bc=hh.BC_PERIODIC,
-                                alpha=-1.0, xmin=-1.0, xmax=1.0, arith=args.arith), order=3)
+    prob = hh.Burgers1D(c.n, 3, 1e-6, flux=hh.LAX_FRIEDRICHS, bc=hh.BC_PERIODIC, alpha=-1.0,
+                        xmin=-1.0, xmax=1.0, arith=args.arith)
+    prob.tune(args.variant, args.cells_per_lane)
+    ode = hh.RkTvd(prob, order=3)
     u = torch.as_tensor(c.u0, device=dev).clone()
     t = 0.0
     if args.warmup > 0:
@@ -148,7 +153,9 @@ def main() -> int:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax>" % args.arith,
+                "kernel": (("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%d>" % args.cells_per_lane)
+                           if args.arith == "fast" and args.variant == 0 else
+                           "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith),
                 "avg_launch_us": round(launch_ms * 1e3, 2),
                 "algorithmic_bytes_per_launch": bytes_per_launch}
 
@@ -165,7 +172,8 @@ def main() -> int:
         "config": {"workload": "burgers1d_weno5_lf_rk3_periodic_2^24cells", "cells": c.n,
                    "k": 3, "eps": 1e-6, "flux": "lax_friedrichs, alpha=max|u| per stage",
                    "integrator": "rktvd order 3", "cfl": 0.4, "dt": c.dt,
-                   "arith": args.arith, "parallelism": "single GPU"},
+                   "arith": args.arith, "variant": args.variant,
+                   "cells_per_lane": args.cells_per_lane, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <string>
 #include <utility>
@@ -17,6 +18,7 @@
 
 #include "../../include/hrweno_capi.h"
 #include "kernels_1d.hpp"
+#include "kernels_1d_fast.hpp"
 #include "kernels_2d.hpp"
 
 using namespace hrw;
@@ -88,6 +90,14 @@ static unsigned grid_for(long long n, int per_block, unsigned cap = 1u << 20)
 // ------------------------------------------------------------------ runtime -----------
 extern "C" const char *hrw_version(void) { return "hrweno-mi355x 0.1.0 (gfx950)"; }
 extern "C" const char *hrw_last_error(void) { return g_err.c_str(); }
+extern "C" int hrw_last_error_copy(char *buf, int64_t len)
+{
+    if (!buf || len < 1) return HRW_ERR_ARG;
+    const size_t n = std::min((size_t)(len - 1), g_err.size());
+    std::memcpy(buf, g_err.data(), n);
+    buf[n] = '\0';
+    return HRW_OK;
+}
 
 extern "C" int hrw_device_count(int *count)
 {
@@ -367,7 +377,6 @@ extern "C" int hrw_weno_calc_cnu(int ncells, int k, const double *xedges, double
 {
     CALL(check_weno_args(ncells, k, 1.0, HRW_ARITH_EXACT));
     if (!xedges || !cnu_host) return fail(HRW_ERR_ARG, "null array");
-    if (ncells < 2) return fail(HRW_ERR_ARG, "Invalid input 'xedges': a non-uniform grid needs ncells >= 2.");
     const size_t count = (size_t)ncells * k * (k + 1);
     CALL(g_stage.reserve(count));
     CALL(compute_cnu(ncells, k, xedges, g_stage.buf));
@@ -381,8 +390,6 @@ extern "C" int hrw_weno_create(int ncells, int k, double eps, const double *xedg
     if (!out) return fail(HRW_ERR_ARG, "hrw_weno_create: null output pointer");
     *out = nullptr;
     CALL(check_weno_args(ncells, k, eps, arith));          // src/hrweno_weno.f90:72-98
-    if (xedges && ncells < 2)
-        return fail(HRW_ERR_ARG, "Invalid input 'xedges': a non-uniform grid needs ncells >= 2.");
     auto *w = new (std::nothrow) hrw_weno_s;
     if (!w) return fail(HRW_ERR_ALLOC, "out of host memory");
     w->n = ncells;
@@ -567,6 +574,8 @@ struct hrw_problem_s {
     // 2D
     int n1 = 0, n2 = 0, ty = 16;
     double x1min = 0, rx1 = 0, x2min = 0, rx2 = 0, g1a = 0, g1b = 0, g2a = 0, g2b = 0;
+    // kernel choice for the 1D FAST form: 0 = register-blocked (cells_per_lane), 1 = LDS tile
+    int variant = 0, cells_per_lane = 8;
     // alpha slot sets for rhs-only calls
     unsigned long long *slots = nullptr;
     bool alphamax() const { return kind == PROB_BURGERS1D && numflux == HRW_FLUX_LAX_FRIEDRICHS && alpha < 0; }
@@ -662,6 +671,17 @@ extern "C" int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, doub
     return HRW_OK;
 }
 
+extern "C" int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane)
+{
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    if (variant != 0 && variant != 1) return fail(HRW_ERR_ARG, "variant must be 0 or 1");
+    if (cells_per_lane != 4 && cells_per_lane != 8)
+        return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
+    p->variant = variant;
+    p->cells_per_lane = cells_per_lane;
+    return HRW_OK;
+}
+
 extern "C" int hrw_problem_destroy(hrw_problem_t p)
 {
     delete p;
@@ -710,8 +730,54 @@ static void l1d_x(int flux, bool per, bool fast, bool amax, const Stage1DArgs &a
     if (flux == LF) l1d_p<K, LF>(per, fast, amax, a, s);
     else l1d_p<K, GODUNOV>(per, fast, amax, a, s);
 }
+template <int K, int FLUX, bool PER, bool AMAX, int R>
+static void l1f(const Stage1DArgs &a, hipStream_t s)
+{
+    constexpr long long per_block = 64LL * R * (kBlock / 64);
+    const unsigned grid = (unsigned)((a.n + per_block - 1) / per_block);
+    stage1d_fast_kernel<K, FLUX, PER, AMAX, R><<<grid, kBlock, 0, s>>>(a);
+}
+template <int K, int FLUX, bool PER, bool AMAX>
+static void l1f_r(int r, const Stage1DArgs &a, hipStream_t s)
+{
+    if (r == 4) l1f<K, FLUX, PER, AMAX, 4>(a, s);
+    else l1f<K, FLUX, PER, AMAX, 8>(a, s);
+}
+template <int K, int FLUX, bool PER>
+static void l1f_a(bool amax, int r, const Stage1DArgs &a, hipStream_t s)
+{
+    if constexpr (FLUX == LF) {
+        if (amax) return l1f_r<K, FLUX, PER, true>(r, a, s);
+    }
+    l1f_r<K, FLUX, PER, false>(r, a, s);
+}
+template <int K>
+static void l1f_x(int flux, bool per, bool amax, int r, const Stage1DArgs &a, hipStream_t s)
+{
+    if (flux == LF) {
+        if (per) l1f_a<K, LF, true>(amax, r, a, s);
+        else l1f_a<K, LF, false>(amax, r, a, s);
+    } else {
+        if (per) l1f_a<K, GODUNOV, true>(amax, r, a, s);
+        else l1f_a<K, GODUNOV, false>(amax, r, a, s);
+    }
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
 {
+    const bool amax0 = p->alphamax();
+    if (p->arith == HRW_ARITH_FAST && p->variant == 0 && a.n < (1LL << 31) - (1LL << 20) &&
+        aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
+        aligned16(a.Lo4) && aligned16(a.Lout)) {
+        const bool per = p->bc == HRW_BC_PERIODIC;
+        const int r = p->cells_per_lane;
+        if (p->k == 1) l1f_x<1>(p->numflux, per, amax0, r, a, s);
+        else if (p->k == 2) l1f_x<2>(p->numflux, per, amax0, r, a, s);
+        else l1f_x<3>(p->numflux, per, amax0, r, a, s);
+        return;
+    }
     const bool per = p->bc == HRW_BC_PERIODIC, fast = p->arith == HRW_ARITH_FAST;
     const bool amax = p->alphamax();
     if (p->k == 1) l1d_x<1>(p->numflux, per, fast, amax, a, s);

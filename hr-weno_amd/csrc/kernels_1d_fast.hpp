@@ -1,0 +1,326 @@
+// kernels_1d_fast.hpp -- register-blocked fused stage kernel for the FAST arithmetic form.
+//
+// The stage pass is bound by FP64 instruction issue, not by HBM (its traffic is already the
+// algorithmic 16/24/24 bytes per cell).  This kernel cuts the instructions per cell:
+//
+//  * each lane owns R consecutive cells; the K-1... halo comes from the neighbouring lanes by
+//    ds_bpermute shuffles (LDS pipe, no VALU), lanes 0 and 63 load theirs directly;
+//  * adjacent cells share work: with D = v0 - 2v1 + v2 and E = v0 - v2 on the triple
+//    T(m) = (v_m, v_m+1, v_m+2), the three smoothness indicators drawing on T(m) are
+//        12*beta_0(m)   = 13 D^2 + 3 (E + 2D)^2
+//        12*beta_1(m+1) = 13 D^2 + 3 E^2
+//        12*beta_2(m+2) = 13 D^2 + 3 (2D - E)^2
+//    and the four candidate values are 6*c(:,r).T(m), r = -1..2 (used as vlr0(m), vrr0(m) =
+//    vlr1(m+1), vrr1(m+1) = vlr2(m+2), vrr2(m+2)); the factors 12 and 6 cancel in the
+//    normalised weights (eps is scaled by 12 to match);
+//  * one Lax-Friedrichs/Godunov flux and one grid edge per interface;
+//  * weights over a common denominator, w_r ~ d_r prod_{s!=r} q_s, with d scaled to integers
+//    (3, 6, 1) so they need three multiplies, and one v_rcp_f64 + two Newton steps per face
+//    instead of IEEE divides (also for 1/width).
+//
+// All of this is an exact rewrite of the same algebra; results differ from the reference's
+// evaluation order by a few ulp per reconstruction (the FAST contract).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels_1d.hpp"
+
+namespace hrw {
+
+// 1/x to ~1 ulp: hardware estimate + two Newton steps.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma_(-x, r, 1.0);
+    r = fma_(r, e, r);
+    e = fma_(-x, r, 1.0);
+    return fma_(r, e, r);
+}
+
+// Shared pieces of the triple (a, b, c) = (v_m, v_m+1, v_m+2) for K = 3.
+struct Tri3 {
+    double bA, bB, bC;       // 12*beta_0(m), 12*beta_1(m+1), 12*beta_2(m+2)
+    double Pm, P0, P1, P2;   // 6*c(:,-1).T, 6*c(:,0).T, 6*c(:,1).T, 6*c(:,2).T
+};
+
+__device__ __forceinline__ Tri3 tri3(double a, double b, double c)
+{
+    Tri3 t;
+    const double D = fma_(-2.0, b, a) + c;
+    const double E = a - c;
+    const double dd = (13.0 * D) * D;
+    t.bB = fma_(3.0 * E, E, dd);
+    const double X = fma_(2.0, D, E);
+    t.bA = fma_(3.0 * X, X, dd);
+    const double Y = fma_(2.0, D, -E);
+    t.bC = fma_(3.0 * Y, Y, dd);
+    t.P0 = fma_(5.0, b, fma_(2.0, a, -c));          // 2a + 5b - c
+    t.P1 = fma_(5.0, b, fma_(2.0, c, -a));          // -a + 5b + 2c
+    t.Pm = fma_(11.0, a, fma_(-7.0, b, c + c));     // 11a - 7b + 2c
+    t.P2 = fma_(11.0, c, fma_(-7.0, b, a + a));     // 2a - 7b + 11c
+    return t;
+}
+
+// One reconstruction from the pieces of T(j-2), T(j-1), T(j).  eps12 = 12*eps.  Each face
+// takes one reciprocal of its own sum of weights, so the only products are pairs of q's: finite
+// while beta < ~1e77, i.e. for cell-value jumps below ~1e38.
+__device__ __forceinline__ void recon3(double eps12, const Tri3 &t2, const Tri3 &t1,
+                                       const Tri3 &t0, double &vl, double &vr)
+{
+    double q0 = eps12 + t0.bA, q1 = eps12 + t1.bB, q2 = eps12 + t2.bC;
+    q0 = q0 * q0;
+    q1 = q1 * q1;
+    q2 = q2 * q2;
+    const double m0 = q1 * q2, m1 = q0 * q2, m2 = q0 * q1;
+    // d = (0.3, 0.6, 0.1) ~ (3, 6, 1) on the right face, reversed on the left face
+    const double a0 = 3.0 * m0, a1 = 6.0 * m1, l2 = 3.0 * m2;
+    const double sr = 6.0 * (a0 + a1 + m2), sl = 6.0 * (m0 + a1 + l2);
+    const double nr = fma_(a0, t0.P0, fma_(a1, t1.P1, m2 * t2.P2));
+    const double nl = fma_(m0, t0.Pm, fma_(a1, t1.P0, l2 * t2.P1));
+    vr = nr * rcp_nr(sr);
+    vl = nl * rcp_nr(sl);
+}
+
+// K = 2: pair (a, b) = (v_m, v_m+1) gives beta_0(m) = beta_1(m+1) = (b-a)^2 and the
+// candidates 2*vrr0(m) = a+b = 2*vlr1(m+1), 2*vrr1(m+1) = 3b-a, 2*vlr0(m) = 3a-b.
+struct Pair2 {
+    double B, S, Rr, Lr;
+};
+__device__ __forceinline__ Pair2 pair2(double a, double b)
+{
+    Pair2 p;
+    const double d = b - a;
+    p.B = d * d;
+    p.S = a + b;
+    p.Rr = fma_(3.0, b, -a);
+    p.Lr = fma_(3.0, a, -b);
+    return p;
+}
+__device__ __forceinline__ void recon2(double eps, const Pair2 &p1, const Pair2 &p0, double &vl,
+                                       double &vr)
+{
+    double q0 = eps + p0.B, q1 = eps + p1.B;
+    q0 = q0 * q0;
+    q1 = q1 * q1;
+    // right: d = (2/3, 1/3) ~ (2, 1): weights ~ 2 q1, q0; left: (1, 2): q1, 2 q0
+    const double a0 = 2.0 * q1, b1 = 2.0 * q0;
+    const double sr = 2.0 * (a0 + q0), sl = 2.0 * (q1 + b1);
+    const double nr = fma_(a0, p0.S, q0 * p1.Rr);
+    const double nl = fma_(q1, p0.Lr, b1 * p1.S);
+    vr = nr * rcp_nr(sr);
+    vl = nl * rcp_nr(sl);
+}
+
+// --------------------------------------------------------------------------------------
+// R consecutive doubles from p (16-byte aligned when R is even) into out.
+template <int R> __device__ __forceinline__ void load_run(const double *p, double *out)
+{
+    if constexpr (R % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < R; i += 2) {
+            const double2 t = *reinterpret_cast<const double2 *>(p + i);
+            out[i] = t.x;
+            out[i + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) out[i] = p[i];
+    }
+}
+template <int R> __device__ __forceinline__ void store_run(double *p, const double *in)
+{
+    if constexpr (R % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < R; i += 2) *reinterpret_cast<double2 *>(p + i) = make_double2(in[i], in[i + 1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) p[i] = in[i];
+    }
+}
+
+// Divergences L of the lane's R cells from v[i] <-> cell c0-H+i.
+template <int K, int FLUX, bool PERIODIC, int R>
+__device__ __forceinline__ void lane_divergence(const Stage1DArgs &a, const double *v, int c0,
+                                                int n, bool interior, double alpha, double *Lv)
+{
+    double fl = 0.0, vr_prev = 0.0;
+    double e_prev = linear_edge32(a.xmin, a.rx, c0 < n ? c0 : n);
+    const double eps12 = 12.0 * a.eps;
+    [[maybe_unused]] Tri3 t2, t1;
+    [[maybe_unused]] Pair2 p1;
+    if constexpr (K == 3) {
+        t2 = tri3(v[0], v[1], v[2]);
+        t1 = tri3(v[1], v[2], v[3]);
+    } else if constexpr (K == 2) {
+        p1 = pair2(v[0], v[1]);
+    }
+    // reconstruction of cell c0-1+s for s = 0..R+1
+#pragma unroll
+    for (int s = 0; s <= R + 1; ++s) {
+        double vl, vr;
+        if constexpr (K == 3) {
+            const Tri3 t0 = tri3(v[s + 2], v[s + 3], v[s + 4]);   // T(c0-1+s)
+            recon3(eps12, t2, t1, t0, vl, vr);
+            t2 = t1;
+            t1 = t0;
+        } else if constexpr (K == 2) {
+            const Pair2 p0 = pair2(v[s + 1], v[s + 2]);
+            recon2(a.eps, p1, p0, vl, vr);
+            p1 = p0;
+        } else {
+            vl = vr = v[s];
+        }
+        if (s == 0) {
+            vr_prev = vr;
+            continue;
+        }
+        // interface (c0-1+s) - 1/2 between cells c0-2+s and c0-1+s
+        const double f = burgers_numflux<FLUX, true>(vr_prev, vl, alpha);
+        vr_prev = vr;
+        if (s == 1) {
+            fl = f;
+            continue;
+        }
+        // cell g = c0 + s - 2 between fluxes fleft and fr
+        const int i = s - 2;
+        const int g = c0 + i;
+        double fr = f, fleft = fl;
+        fl = f;
+        double w;
+        if (a.width) {
+            w = a.width[g < n ? g : n - 1];
+        } else {
+            const double e = linear_edge32(a.xmin, a.rx, g + 1);
+            w = e - e_prev;
+            e_prev = e;
+        }
+        if constexpr (!PERIODIC) {
+            if (!interior) {   // example1:103-104
+                if (g == 0) fleft = fr;
+                if (g == n - 1) fr = fleft;
+            }
+        }
+        Lv[i] = (fleft - fr) * rcp_nr(w);
+    }
+}
+
+// Wave w of block b owns cells [W, W + 64R), W = (4b + w)*64R; lane l owns R cells from
+// c0 = W + l*R.  Halo: H = K cells on each side (K-1 for the reconstruction window, one more
+// for the neighbouring reconstruction that the edge interface needs).  Interior waves (the
+// whole range and halo inside the line) take vector loads/stores and no boundary checks.
+// Requires n < 2^31 and 16-byte aligned arrays (checked on the host).
+#ifndef HRW_FAST_WAVES_PER_EU
+#define HRW_FAST_WAVES_PER_EU 4
+#endif
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_FAST_WAVES_PER_EU)))
+void stage1d_fast_kernel(Stage1DArgs a)
+{
+    constexpr int H = K;
+    constexpr int NV = R + 2 * H;
+    __shared__ double s_red[kBlock / 64 + 1];
+    const int lane = threadIdx.x & 63;
+    const int n = (int)a.n;
+    const int W = (int)(((long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64LL * R));
+    const int c0 = W + lane * R;
+
+    double alpha = a.alpha;
+    if constexpr (ALPHAMAX) {
+        // every wave reduces the 64 partial maxima itself (no barrier)
+        alpha = wave_max(__longlong_as_double((long long)a.alpha_in[lane * kSlotStride]));
+    }
+    clear_set(a.alpha_clear);
+
+    // ---- phase 1: the lane's cells and halo (v[i] <-> cell c0 - H + i), then L
+    double v[NV];
+    const bool interior = W >= H && (long long)W + 64LL * R + H <= n;
+    if (interior) {
+        load_run<R>(a.us + c0, v + H);
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            v[i] = __shfl_up(v[R + i], 1, 64);            // lane-1's last H cells
+            v[H + R + i] = __shfl_down(v[H + i], 1, 64);  // lane+1's first H cells
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) v[i] = a.us[c0 - H + i];
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) v[H + R + i] = a.us[c0 + R + i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = a.us[cell_index(c0 - H + i, n, PERIODIC)];
+    }
+    double Lv[R];
+    lane_divergence<K, FLUX, PERIODIC, R>(a, v, c0, n, interior, alpha, Lv);
+
+    // ---- phase 2: stage combination with the pointwise operands, then write back
+    double un[R];
+    const double *us = v + H;
+    const int stage = a.stage;
+    const double h = a.h;
+    double pw[R], pw2[R];
+    auto load_pw = [&](const double *src, double *dst) {
+        if (interior) {
+            load_run<R>(src + c0, dst);
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) dst[i] = src[c0 + i < n ? c0 + i : n - 1];
+        }
+    };
+    switch (stage) {
+    case RK1:
+    case RK2_1:
+    case RK3_1:
+#pragma unroll
+        for (int i = 0; i < R; ++i) un[i] = fma_(h, Lv[i], us[i]);
+        break;
+    case RK2_2:
+        load_pw(a.u0, pw);
+#pragma unroll
+        for (int i = 0; i < R; ++i) un[i] = 0.5 * fma_(h, Lv[i], pw[i] + us[i]);
+        break;
+    case RK3_2:
+        load_pw(a.u0, pw);
+#pragma unroll
+        for (int i = 0; i < R; ++i) un[i] = 0.25 * fma_(h, Lv[i], fma_(3.0, pw[i], us[i]));
+        break;
+    case RK3_3:
+        load_pw(a.u0, pw);
+#pragma unroll
+        for (int i = 0; i < R; ++i) un[i] = div3_fast(fma_(2.0 * h, Lv[i], fma_(2.0, us[i], pw[i])));
+        break;
+    case MS3:
+        load_pw(a.uo4, pw);
+        load_pw(a.Lo4, pw2);
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            un[i] = 0.03125 * fma_(10.0 * h, pw2[i], fma_(7.0, pw[i], fma_(50.0 * h, Lv[i], 25.0 * us[i])));
+        break;
+    default:   // rhs only: the next launch's alpha is max|us|
+#pragma unroll
+        for (int i = 0; i < R; ++i) un[i] = us[i];
+        break;
+    }
+    double m_out = 0.0;
+    if (interior) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) m_out = fmax(m_out, fabs(un[i]));
+        if (stage >= 0) store_run<R>(a.uo + c0, un);
+        if (a.Lout) store_run<R>(a.Lout + c0, Lv);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int g = c0 + i;
+            if (g >= n) continue;   // cells past the end are not part of the state
+            m_out = fmax(m_out, fabs(un[i]));
+            if (stage >= 0) a.uo[g] = un[i];
+            if (a.Lout) a.Lout[g] = Lv[i];
+        }
+    }
+    if constexpr (ALPHAMAX) publish_max(m_out, a.alpha_out, s_red);
+}
+
+}  // namespace hrw

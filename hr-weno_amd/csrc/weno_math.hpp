@@ -235,6 +235,15 @@ HRW_HD double linear_edge(double xmin, double rx, long long i)
     return xmin + rx * (double)i;
 #endif
 }
+// The same for 32-bit indices (one conversion instruction instead of four).
+HRW_HD double linear_edge32(double xmin, double rx, int i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __dadd_rn(xmin, __dmul_rn(rx, (double)i));
+#else
+    return xmin + rx * (double)i;
+#endif
+}
 
 // x/3 correctly rounded except in rare ties, from one multiply and two FMAs.
 HRW_HD double div3_fast(double x)

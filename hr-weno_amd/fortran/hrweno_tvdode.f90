@@ -1,0 +1,258 @@
+module hrweno_tvdode
+!! TVD Runge-Kutta (orders 1-3) and 5-step third-order multistep integrators with the public
+!! API of the reference module hrweno_tvdode (src/hrweno_tvdode.f90): `rktvd(fu, neq, order)`,
+!! `mstvd(fu, neq)` and `%integrate(u, t, tout, dt[, itask])`, the fields neq, order, fevals,
+!! istate and msg, and the reference's loop control (strict is_done, so the last step may
+!! overshoot tout).
+!!
+!! The time loop and the stage combinations run on the MI355X (libhrweno_hip):
+!!  * rktvd(fu, neq, order) / mstvd(fu, neq): `fu` is the user's host routine.  The state
+!!    lives on the device; before each call of fu it is copied into a host buffer and udot is
+!!    copied back afterwards.
+!!  * rktvd(prob, order) / mstvd(prob) with a device problem descriptor (hrweno_problems): the
+!!    RHS is fused with each stage update, one device pass per stage.
+!! `%integrate` takes a host array (copied in and out once per call); `%integrate_dev` takes
+!! a device array and copies nothing.
+   use, intrinsic :: iso_c_binding
+   use hrweno_kinds, only: rk
+   use hrweno_capi
+   use hrweno_problems, only: problem
+   implicit none
+   private
+
+   public :: rktvd, mstvd, integrand
+
+   abstract interface
+      subroutine integrand(t, u, udot)
+      !! Integrand u'(t, u) (src/hrweno_tvdode.f90:50-57).
+         import :: rk
+         real(rk), intent(in) :: t, u(:)
+         real(rk), intent(out) :: udot(:)
+      end subroutine
+   end interface
+
+   type :: callback_ctx
+      procedure(integrand), pointer, nopass :: fu => null()
+      integer :: neq = 0
+      real(rk), allocatable :: u(:), udot(:)
+   end type callback_ctx
+
+   type, abstract :: tvdode
+   !! Common part of the TVD solvers (src/hrweno_tvdode.f90:14-34).
+      procedure(integrand), pointer, nopass, private :: fu => null()
+         !! user integrand (callback form)
+      integer :: neq
+         !! number of equations
+      integer :: order
+         !! order of the method
+      integer :: fevals = 0
+         !! number of function evaluations
+      integer :: istate = 0
+         !! 1 first call for a problem, 2 subsequent call, -1 error
+      character(:), allocatable :: msg
+         !! error message
+      type(c_ptr), private :: h = c_null_ptr
+         !! library ODE object
+      type(c_ptr), private :: ubuf = c_null_ptr
+         !! device copy of u for host-array calls
+      type(callback_ctx), pointer, private :: ctx => null()
+   contains
+      procedure, pass(self) :: error_msg
+      procedure, pass(self) :: integrate_dev => tvdode_integrate_dev
+   end type tvdode
+
+   type, extends(tvdode) :: rktvd
+   !! Runge-Kutta TVD solver.
+   contains
+      procedure, pass(self) :: integrate => rktvd_integrate
+   end type rktvd
+
+   type, extends(tvdode) :: mstvd
+   !! Multistep TVD solver.
+   contains
+      procedure, pass(self) :: integrate => mstvd_integrate
+   end type mstvd
+
+   interface rktvd
+      module procedure :: rktvd_init, rktvd_init_problem
+   end interface rktvd
+
+   interface mstvd
+      module procedure :: mstvd_init, mstvd_init_problem
+   end interface mstvd
+
+contains
+
+   integer(c_int) function host_callback(ctx, t, u, udot, stream) bind(C)
+   !! Calls the user's host integrand on host copies of the device arrays.
+      type(c_ptr), value :: ctx, u, udot, stream
+      real(c_double), value :: t
+      type(callback_ctx), pointer :: c
+      integer(c_size_t) :: bytes
+      call c_f_pointer(ctx, c)
+      bytes = int(c%neq, c_size_t)*8_c_size_t
+      host_callback = hrw_memcpy_dtoh(c_loc(c%u), u, bytes, stream)
+      if (host_callback /= 0) return
+      call c%fu(t, c%u, c%udot)
+      host_callback = hrw_memcpy_htod(udot, c_loc(c%udot), bytes, stream)
+   end function host_callback
+
+   subroutine setup_callback(self, fu, neq)
+      class(tvdode), intent(inout) :: self
+      procedure(integrand) :: fu
+      integer, intent(in) :: neq
+      self%fu => fu
+      allocate (self%ctx)
+      self%ctx%fu => fu
+      self%ctx%neq = neq
+      allocate (self%ctx%u(neq), self%ctx%udot(neq))
+   end subroutine setup_callback
+
+   subroutine finish_init(self, rc)
+      class(tvdode), intent(inout) :: self
+      integer(c_int), intent(in) :: rc
+      if (rc /= 0) call self%error_msg(hrw_message())
+      if (hrw_device_alloc(int(self%neq, c_size_t)*8_c_size_t, self%ubuf) /= 0) &
+         call self%error_msg(hrw_message())
+      self%istate = 1
+   end subroutine finish_init
+
+   type(rktvd) function rktvd_init(fu, neq, order) result(self)
+   !! rktvd(fu, neq, order) (src/hrweno_tvdode.f90:69-95).
+      procedure(integrand) :: fu
+      integer, intent(in) :: neq
+      integer, intent(in) :: order
+      integer(c_int) :: rc
+      if (neq > 0) then
+         self%neq = neq
+      else
+         call self%error_msg("Invalid input 'neq'. Valid range: neq >= 1.")
+      end if
+      if ((order >= 1) .and. (order <= 3)) then
+         self%order = order
+      else
+         call self%error_msg("Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.")
+      end if
+      call setup_callback(self, fu, neq)
+      rc = hrw_rktvd_create_cb(c_funloc(host_callback), c_loc(self%ctx), int(neq, c_int64_t), &
+                               int(order, c_int), HRW_ARITH_EXACT, self%h)
+      call finish_init(self, rc)
+   end function rktvd_init
+
+   type(rktvd) function rktvd_init_problem(prob, order) result(self)
+   !! rktvd on a device problem descriptor: fused RHS + stage passes.
+      type(problem), intent(in) :: prob
+      integer, intent(in) :: order
+      integer(c_int) :: rc
+      self%neq = prob%neq
+      if ((order >= 1) .and. (order <= 3)) then
+         self%order = order
+      else
+         call self%error_msg("Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.")
+      end if
+      rc = hrw_rktvd_create(prob%h, int(order, c_int), self%h)
+      call finish_init(self, rc)
+   end function rktvd_init_problem
+
+   type(mstvd) function mstvd_init(fu, neq) result(self)
+   !! mstvd(fu, neq) (src/hrweno_tvdode.f90:180-201).
+      procedure(integrand) :: fu
+      integer, intent(in) :: neq
+      integer(c_int) :: rc
+      if (neq > 0) then
+         self%neq = neq
+      else
+         call self%error_msg("Invalid input 'neq'. Valid range: neq >= 1.")
+      end if
+      self%order = 3
+      call setup_callback(self, fu, neq)
+      rc = hrw_mstvd_create_cb(c_funloc(host_callback), c_loc(self%ctx), int(neq, c_int64_t), &
+                               HRW_ARITH_EXACT, self%h)
+      call finish_init(self, rc)
+   end function mstvd_init
+
+   type(mstvd) function mstvd_init_problem(prob) result(self)
+   !! mstvd on a device problem descriptor.
+      type(problem), intent(in) :: prob
+      integer(c_int) :: rc
+      self%neq = prob%neq
+      self%order = 3
+      rc = hrw_mstvd_create(prob%h, self%h)
+      call finish_init(self, rc)
+   end function mstvd_init_problem
+
+   subroutine tvdode_integrate_dev(self, u, t, tout, dt, itask)
+   !! %integrate on a device array u (neq doubles); no host copies.
+      class(tvdode), intent(inout) :: self
+      type(c_ptr), intent(in) :: u
+      real(rk), intent(inout) :: t
+      real(rk), intent(in) :: tout, dt
+      integer, intent(in), optional :: itask
+      integer(c_int) :: task, rc, order, fe, st
+      integer(c_int64_t) :: neq
+      if (self%istate < 1) return
+      task = 1
+      if (present(itask)) task = itask
+      rc = hrw_ode_integrate(self%h, u, t, tout, dt, task)
+      if (rc /= 0) call self%error_msg(hrw_message())
+      rc = hrw_ode_info(self%h, neq, order, fe, st)
+      self%fevals = fe
+      self%istate = st
+   end subroutine tvdode_integrate_dev
+
+   subroutine integrate_host(self, u, t, tout, dt, itask)
+      class(tvdode), intent(inout) :: self
+      real(rk), intent(inout) :: u(:)
+      real(rk), intent(inout) :: t
+      real(rk), intent(in) :: tout, dt
+      integer, intent(in) :: itask
+      real(rk), allocatable, target :: w(:)
+      integer(c_size_t) :: bytes
+      if (self%istate < 1) return
+      if (size(u) /= self%neq) call self%error_msg("Invalid input 'u': size(u) /= neq.")
+      w = u
+      bytes = int(self%neq, c_size_t)*8_c_size_t
+      if (hrw_memcpy_htod(self%ubuf, c_loc(w), bytes, c_null_ptr) /= 0) &
+         call self%error_msg(hrw_message())
+      call self%integrate_dev(self%ubuf, t, tout, dt, itask)
+      if (hrw_memcpy_dtoh(c_loc(w), self%ubuf, bytes, c_null_ptr) /= 0) &
+         call self%error_msg(hrw_message())
+      u = w
+   end subroutine integrate_host
+
+   subroutine rktvd_integrate(self, u, t, tout, dt, itask)
+   !! Optimal TVD RK of order 1-3 to tout, or one step if itask = 2
+   !! (src/hrweno_tvdode.f90:97-178).
+      class(rktvd), intent(inout) :: self
+      real(rk), intent(inout) :: u(:)
+      real(rk), intent(inout) :: t
+      real(rk), intent(in) :: tout
+      real(rk), intent(in) :: dt
+      integer, intent(in), optional :: itask
+      integer :: task
+      task = 1
+      if (present(itask)) task = itask
+      call integrate_host(self, u, t, tout, dt, task)
+   end subroutine rktvd_integrate
+
+   subroutine mstvd_integrate(self, u, t, tout, dt)
+   !! 5-step third-order TVD multistep to tout, started by four RK3 steps on the first call
+   !! (src/hrweno_tvdode.f90:203-271).
+      class(mstvd), intent(inout) :: self
+      real(rk), intent(inout) :: u(:)
+      real(rk), intent(inout) :: t
+      real(rk), intent(in) :: tout
+      real(rk), intent(in) :: dt
+      call integrate_host(self, u, t, tout, dt, 1)
+   end subroutine mstvd_integrate
+
+   pure subroutine error_msg(self, msg)
+   !! Error method (src/hrweno_tvdode.f90:286-297).
+      class(tvdode), intent(inout) :: self
+      character(*), intent(in) :: msg
+      self%msg = msg
+      self%istate = -1
+      error stop self%msg
+   end subroutine error_msg
+
+end module hrweno_tvdode

@@ -25,6 +25,7 @@ INTEGRAND = C.CFUNCTYPE(C.c_int, P, D, P, P, P)
 SIGNATURES = {
     "hrw_version": [],
     "hrw_last_error": [],
+    "hrw_last_error_copy": [C.c_char_p, L],
     "hrw_device_count": [PI],
     "hrw_device_alloc": [C.c_size_t, C.POINTER(P)],
     "hrw_device_free": [P],
@@ -56,6 +57,7 @@ SIGNATURES = {
     "hrw_grid_compute": [I, PD, PD, PD],
     "hrw_problem_burgers1d": [I, I, D, I, I, D, D, D, PD, I, C.POINTER(P)],
     "hrw_problem_pbe2d": [I, I, I, D, I, D, D, D, D, D, D, D, D, I, C.POINTER(P)],
+    "hrw_problem_tune": [P, I, I],
     "hrw_problem_destroy": [P],
     "hrw_problem_neq": [P, PL],
     "hrw_problem_rhs": [P, D, P, P, P],

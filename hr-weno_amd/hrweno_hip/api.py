@@ -225,6 +225,11 @@ class Problem:
         call("hrw_problem_neq", self._h, C.byref(n))
         return n.value
 
+    def tune(self, variant: int = 0, cells_per_lane: int = 8):
+        """Kernel choice of the 1D FAST form (0: register-blocked, 1: LDS tile)."""
+        call("hrw_problem_tune", self._h, int(variant), int(cells_per_lane))
+        return self
+
     def rhs(self, t, u, udot=None):
         torch = _torch()
         if udot is None:

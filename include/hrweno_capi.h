@@ -59,6 +59,8 @@ enum hrw_bc {
 const char *hrw_version(void);
 /* Message of the last failed call on this thread ("" if none). */
 const char *hrw_last_error(void);
+/* The same message copied into buf (NUL-terminated, truncated to len-1 characters). */
+int hrw_last_error_copy(char *buf, int64_t len);
 int hrw_device_count(int *count);
 int hrw_device_alloc(size_t bytes, void **ptr);
 int hrw_device_free(void *ptr);
@@ -149,6 +151,10 @@ int hrw_problem_burgers1d(int n, int k, double eps, int numflux, int bc, double 
 int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, double x1max,
                       double x2min, double x2max, double g1a, double g1b, double g2a,
                       double g2b, int arith, hrw_problem_t *out);
+/* Kernel choice for the 1D FAST form (no reference counterpart; used by the autotuner):
+ * variant 0 = register-blocked kernel with cells_per_lane (4 or 8) consecutive cells per lane,
+ * variant 1 = LDS-tile kernel.  Results agree within the FAST tolerance either way. */
+int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane);
 int hrw_problem_destroy(hrw_problem_t p);
 int hrw_problem_neq(hrw_problem_t p, int64_t *neq);
 /* One RHS evaluation udot = L(u) on device arrays (the user `rhs` of the examples). */
