@@ -1,14 +1,20 @@
 #!/usr/bin/env python3
-"""Benchmark of the fused WENO5 + Lax-Friedrichs + SSP-RK3 path (FP64) on one MI355X.
+"""Benchmarks of the fused WENO -> flux -> divergence -> TVD-stage path (FP64) on one MI355X.
 
-Workload: 1D Burgers on 2**24 periodic cells of [-1, 1] (8 seeded sine modes + 16 seeded
-jumps), k = 3, eps = 1e-6, alpha = max|u| of each stage input, fixed dt at CFL 0.4.  A "step"
-is one RK3 time step of the whole line: three fused stage launches.  The state stays resident
-in HBM; only the integrate calls are timed (HIP events on the launch stream).
+Default workload (the headline line): 1D Burgers on 2**24 periodic cells of [-1, 1] (8 seeded
+sine modes + 16 seeded jumps), WENO5 (k = 3, eps = 1e-6), Lax-Friedrichs with alpha = max|u|
+of each stage input, SSP-RK3 at CFL 0.4.  A "step" is one RK3 time step of the whole line:
+three fused stage launches.  The state stays resident in HBM; only the integrate calls are
+timed (HIP events on the launch stream).
 
-Prints ONE JSON line.  `roofline` prices the stage kernel against HBM (64 algorithmic bytes
-per cell-step = 16 + 24 + 24 over the three stages); `cpu_baseline` times the reference's own
-compiled hrweno_weno/hrweno_fluxes (oracle/_ref) on a bounded sample on this host.
+Other workloads (--workload): `batch` -- 4096 independent periodic lines of log-uniform length
+in [256, 65536], k in {2, 3}, own dt, 200 RK3 steps, one launch per stage for all lines (graph
+replayed) vs one launch per line; `pbe` -- 2D population balance on 2048 x 2048 with the
+5-step multistep method, 500 steps.
+
+Prints ONE JSON line.  `roofline` prices the dominant kernel against HBM with the algorithmic
+bytes per cell-step (RK3 1D: 16 + 24 + 24 = 64; multistep 2D: 40); `cpu_baseline` times the
+reference's own compiled hrweno_weno/hrweno_fluxes (oracle/_ref) on a bounded sample.
 """
 from __future__ import annotations
 
@@ -22,10 +28,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "hr-weno_amd"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_STEP = 64.0          # RK3, one fused pass per stage: 16 + 24 + 24
-LAUNCHES_PER_STEP = 3
 METRIC = "FP64 cell-updates/s & % HBM peak: plain vs batched vs adjoint, ≤1e-12 rel"
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "stage1d_burgers_2p24_traffic.json")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.json")
 
 
 def log(msg: str) -> None:
@@ -34,7 +38,7 @@ def log(msg: str) -> None:
 
 def cpu_baseline(seconds: float):
     """The reference's compiled reconstruct + Lax-Friedrichs (oracle/_ref) driving periodic
-    Burgers RK3 on a 2**20-cell sample of the same workload, one core."""
+    Burgers RK3 on a 2**20-cell sample of the headline workload, one core."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     from hrweno_hip import workloads as W
     import oracle as O
@@ -79,18 +83,137 @@ def cpu_baseline(seconds: float):
                       f"{dt:.1f} s on one host core; {what}"}
 
 
+def traffic_for(kernel_key: str):
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(kernel_key, {}).get("hbm_bytes_per_launch")
+    return None
+
+
+def timed(torch, fn):
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def roofline(bytes_per_launch, launch_ms, kernel, key):
+    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(key),
+            "kernel": kernel, "avg_launch_us": round(launch_ms * 1e3, 2),
+            "algorithmic_bytes_per_launch": bytes_per_launch}
+
+
+def run_plain(args, torch, hh, W, dev):
+    log(f"generating workload: {args.cells} cells")
+    c = W.config2(n=args.cells)
+    prob = hh.Burgers1D(c.n, 3, 1e-6, flux=hh.LAX_FRIEDRICHS, bc=hh.BC_PERIODIC, alpha=-1.0,
+                        xmin=-1.0, xmax=1.0, arith=args.arith)
+    prob.tune(args.variant, args.cells_per_lane)
+    ode = hh.RkTvd(prob, order=3)
+    u = torch.as_tensor(c.u0, device=dev).clone()
+    t = 0.0
+    if args.warmup > 0:
+        t = ode.integrate(u, t, W.steps_tout(t, c.dt, args.warmup), c.dt)
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+    fe0 = ode.fevals
+    tout = W.steps_tout(t, c.dt, args.steps)
+    res = {}
+    ms = timed(torch, lambda: res.setdefault("t", ode.integrate(u, t, tout, c.dt)))
+    assert (ode.fevals - fe0) // 3 == args.steps
+    if not bool(torch.isfinite(u).all()):
+        raise RuntimeError("non-finite state after the timed steps")
+    log(f"timed {args.steps} steps: {ms:.3f} ms")
+    fast_v2 = args.arith == "fast" and args.variant == 0
+    kernel = ("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%d>" % args.cells_per_lane
+              if fast_v2 else "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith)
+    rl = roofline(64.0 / 3 * c.n, ms / (3 * args.steps), kernel, "stage1d_2p24")
+    config = {"workload": "burgers1d_weno5_lf_rk3_periodic_2^24cells", "cells": c.n, "k": 3,
+              "eps": 1e-6, "flux": "lax_friedrichs, alpha=max|u| per stage",
+              "integrator": "rktvd order 3", "cfl": 0.4, "dt": c.dt, "arith": args.arith,
+              "variant": args.variant, "cells_per_lane": args.cells_per_lane,
+              "parallelism": "single GPU"}
+    return c.n * args.steps / (ms / 1e3), ms / args.steps, rl, config, \
+        "synthetic (seeded SplitMix64: 8 sine modes + 16 jumps)"
+
+
+def run_batch(args, torch, hh, W, dev):
+    import numpy as np
+    steps = args.steps
+    log("generating workload: 4096 lines")
+    cases = W.config3(steps=steps)
+    lens = np.array([c.n for c in cases])
+    ks = np.array([c.k for c in cases])
+    dts = np.array([c.dt for c in cases])
+    total = int(lens.sum())
+    b = hh.BurgersBatch(lens, ks, dts)
+    u0 = b.pack([c.u0 for c in cases], dev)
+    u = u0.clone()
+    r = b.tune(u)
+    b.step(u, args.warmup, graph=True)
+    ms = timed(torch, lambda: b.step(u, steps, graph=True))
+    log(f"batched (graph, {r} cells/lane): {steps} steps {ms:.3f} ms")
+    ms_nograph = timed(torch, lambda: b.step(u, steps, graph=False))
+    log(f"batched (no graph): {steps} steps {ms_nograph:.3f} ms")
+    # one-launch-per-line path on a bounded number of steps
+    pl_steps = max(1, min(steps, args.per_line_steps))
+    up = u0.clone()
+    ms_pl = timed(torch, lambda: b.step_per_line(up, pl_steps))
+    log(f"per-line: {pl_steps} steps {ms_pl:.3f} ms")
+    value = total * steps / (ms / 1e3)
+    per_line_value = total * pl_steps / (ms_pl / 1e3)
+    rl = roofline(64.0 / 3 * total, ms / (3 * steps),
+                  "hrw::batch_stage_kernel<LF,alphamax,%d>" % r, "batch_c3")
+    config = {"workload": "burgers1d_batch_4096_lines_log_uniform_256_65536", "lines": len(cases),
+              "cells": total, "k": "2 or 3 per line", "integrator": "rktvd order 3",
+              "cells_per_lane": r, "graph": True, "parallelism": "single GPU",
+              "batched_no_graph_cell_updates_per_s": total * steps / (ms_nograph / 1e3),
+              "per_line_cell_updates_per_s": per_line_value, "per_line_steps": pl_steps,
+              "speedup_vs_per_line": value / per_line_value}
+    return value, ms / steps, rl, config, "synthetic (seeded SplitMix64, config-1 style lines)"
+
+
+def run_pbe(args, torch, hh, W, dev):
+    c = W.config4(steps=args.steps)
+    ode = hh.MsTvd(hh.PBE2D(c.n1, c.n2, k=3, bc=hh.BC_INFLOW0_OUTFLOW, arith=args.arith))
+    u = torch.as_tensor(c.u0, device=dev).clone()
+    t = ode.integrate(u, 0.0, W.steps_tout(0.0, c.dt, 4 + args.warmup), c.dt)   # startup + warmup
+    tout = W.steps_tout(t, c.dt, args.steps)
+    fe0 = ode.fevals
+    ms = timed(torch, lambda: ode.integrate(u, t, tout, c.dt))
+    assert ode.fevals - fe0 == args.steps
+    ncell = c.n1 * c.n2
+    rl = roofline(40.0 * ncell, ms / args.steps,
+                  "hrw::pbe2d_kernel<3,inflow0_outflow,%s>" % args.arith, "pbe2d_2048")
+    config = {"workload": "pbe2d_2048x2048_weno5_godunov_mstvd", "cells": ncell,
+              "integrator": "mstvd (5-step, 3rd order)", "g1": "1+0.5*x1", "g2": "0.5",
+              "bc": "zero inflow, upwind outflow", "arith": args.arith,
+              "parallelism": "single GPU"}
+    return ncell * args.steps / (ms / 1e3), ms / args.steps, rl, config, \
+        "synthetic (seeded Gaussian pulse)"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["plain", "batch", "pbe"], default="plain")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cells", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0,
                     help="1D FAST kernel: 0 register-blocked, 1 LDS tile")
     ap.add_argument("--cells-per-lane", type=int, default=8)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--per-line-steps", type=int, default=5)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of --cells doubles (20 times), the "
                          "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters")
@@ -99,6 +222,8 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards "
               "by domain decomposition, which this build does not launch.", file=sys.stderr)
         return 2
+    if args.steps is None:
+        args.steps = {"plain": 100, "batch": 200, "pbe": 500}[args.workload]
 
     import torch
     import hrweno_hip as hh
@@ -113,69 +238,20 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy probe: 20 copies of {args.cells * 8} bytes")
         return 0
-    log(f"generating workload: {args.cells} cells")
-    c = W.config2(n=args.cells)
-    prob = hh.Burgers1D(c.n, 3, 1e-6, flux=hh.LAX_FRIEDRICHS, bc=hh.BC_PERIODIC, alpha=-1.0,
-                        xmin=-1.0, xmax=1.0, arith=args.arith)
-    prob.tune(args.variant, args.cells_per_lane)
-    ode = hh.RkTvd(prob, order=3)
-    u = torch.as_tensor(c.u0, device=dev).clone()
-    t = 0.0
-    if args.warmup > 0:
-        t = ode.integrate(u, t, W.steps_tout(t, c.dt, args.warmup), c.dt)
-    torch.cuda.synchronize()
-    log(f"warmup done ({args.warmup} steps)")
 
-    stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    fe0 = ode.fevals
-    tout = W.steps_tout(t, c.dt, args.steps)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    t = ode.integrate(u, t, tout, c.dt)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    steps_done = (ode.fevals - fe0) // 3
-    assert steps_done == args.steps, (steps_done, args.steps)
-    if not bool(torch.isfinite(u).all()):
-        raise RuntimeError("non-finite state after the timed steps")
-    log(f"timed {args.steps} steps: {ms:.3f} ms")
-
-    value = c.n * args.steps / (ms / 1e3)
-    launch_ms = ms / (LAUNCHES_PER_STEP * args.steps)
-    bytes_per_launch = BYTES_PER_CELL_STEP / LAUNCHES_PER_STEP * c.n
-    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": (("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%d>" % args.cells_per_lane)
-                           if args.arith == "fast" and args.variant == 0 else
-                           "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith),
-                "avg_launch_us": round(launch_ms * 1e3, 2),
-                "algorithmic_bytes_per_launch": bytes_per_launch}
+    runner = {"plain": run_plain, "batch": run_batch, "pbe": run_pbe}[args.workload]
+    value, ms_step, rl, config, data = runner(args, torch, hh, W, dev)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "plain":
         log("cpu baseline ...")
         cpu = cpu_baseline(args.cpu_seconds)
 
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded SplitMix64: 8 sine modes + 16 jumps)",
-        "config": {"workload": "burgers1d_weno5_lf_rk3_periodic_2^24cells", "cells": c.n,
-                   "k": 3, "eps": 1e-6, "flux": "lax_friedrichs, alpha=max|u| per stage",
-                   "integrator": "rktvd order 3", "cfl": 0.4, "dt": c.dt,
-                   "arith": args.arith, "variant": args.variant,
-                   "cells_per_lane": args.cells_per_lane, "parallelism": "single GPU"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "data": data, "config": config, "roofline": rl, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <memory>
 #include <new>
 #include <string>
 #include <utility>
@@ -1152,4 +1153,348 @@ extern "C" int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tou
                                       : rk_integrate(ode, u, t, tout, dt, itask);
     if (rc != HRW_OK) ode->istate = -1;
     return rc;
+}
+
+// ------------------------------------------------------------------ batched lines -----
+// An ensemble of independent periodic 1D Burgers lines advanced together: one launch per RK3
+// stage covers every line (chunk table, per-line k/dt/grid/alpha), optionally replayed from a
+// captured hipGraph.  The plain path (one set of launches per line) is kept for comparison.
+struct hrw_batch_s {
+    int nlines = 0, numflux = HRW_FLUX_LAX_FRIEDRICHS, arith = HRW_ARITH_FAST, R = 8;
+    double eps = 1e-6, alpha = -1.0;
+    int64_t total = 0;
+    std::vector<int64_t> h_off;
+    std::vector<int> h_n, h_k;
+    std::vector<double> h_dt, h_xmin, h_rx;
+    long long nchunks = 0;
+    int chunk_R = 0;
+    long long *d_off = nullptr;
+    int *d_n = nullptr, *d_k = nullptr, *d_cline = nullptr, *d_cstart = nullptr;
+    double *d_h = nullptr, *d_xmin = nullptr, *d_rx = nullptr;
+    double *ui = nullptr, *ui2 = nullptr;
+    unsigned long long *slots = nullptr;       // 3 sets x nlines
+    unsigned long long *line_slots = nullptr;  // per-line path: nlines x 3 x kNSlot x kSlotStride
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipGraphExec_t graph = nullptr;
+    const double *graph_u = nullptr;
+    int graph_R = 0;
+    bool alphamax() const { return numflux == HRW_FLUX_LAX_FRIEDRICHS && alpha < 0; }
+    ~hrw_batch_s()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        dfree(d_off); dfree(d_n); dfree(d_k); dfree(d_cline); dfree(d_cstart);
+        dfree(d_h); dfree(d_xmin); dfree(d_rx); dfree(ui); dfree(ui2); dfree(slots);
+        dfree(line_slots);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+template <class T> static int upload(T **d, const std::vector<T> &h)
+{
+    CALL(dalloc(d, h.size()));
+    HIP_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return HRW_OK;
+}
+
+// Chunk table for chunks of 64*R cells.
+static int batch_chunks(hrw_batch_s *b, int R)
+{
+    if (b->chunk_R == R) return HRW_OK;
+    std::vector<int> cl, cs;
+    for (int l = 0; l < b->nlines; ++l)
+        for (int st = 0; st < b->h_n[l]; st += 64 * R) {
+            cl.push_back(l);
+            cs.push_back(st);
+        }
+    dfree(b->d_cline);
+    dfree(b->d_cstart);
+    CALL(upload(&b->d_cline, cl));
+    CALL(upload(&b->d_cstart, cs));
+    b->nchunks = (long long)cl.size();
+    b->chunk_R = R;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_burgers_create(int64_t nlines, const int *ncells, const int *k,
+                                        const double *dt, const double *xmin, const double *xmax,
+                                        double eps, int numflux, double alpha, int arith,
+                                        hrw_batch_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (nlines < 1 || nlines > (1 << 30)) return fail(HRW_ERR_ARG, "nlines must be in [1, 2^30]");
+    if (!ncells || !k || !dt) return fail(HRW_ERR_ARG, "null per-line array");
+    if (!(eps > 2.220446049250313e-16))
+        return fail(HRW_ERR_ARG, "Invalid input 'eps'. Valid range: eps > epsilon.");
+    if (numflux != HRW_FLUX_LAX_FRIEDRICHS && numflux != HRW_FLUX_GODUNOV)
+        return fail(HRW_ERR_ARG, "unknown numerical flux %d", numflux);
+    if (arith != HRW_ARITH_FAST)
+        return fail(HRW_ERR_ARG, "the batched path implements the FAST arithmetic form only");
+    auto *b = new (std::nothrow) hrw_batch_s;
+    if (!b) return fail(HRW_ERR_ALLOC, "out of host memory");
+    std::unique_ptr<hrw_batch_s> guard(b);
+    b->nlines = (int)nlines;
+    b->eps = eps;
+    b->numflux = numflux;
+    b->alpha = alpha;
+    b->arith = arith;
+    b->h_off.resize(nlines + 1);
+    int64_t off = 0;
+    for (int64_t l = 0; l < nlines; ++l) {
+        if (!(ncells[l] > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
+        if (!(k[l] >= 1 && k[l] <= 3)) return fail(HRW_ERR_ARG, "Invalid input 'k'. Valid range: 1 <= k <= 3.");
+        if (!(dt[l] != 0.0) || !std::isfinite(dt[l])) return fail(HRW_ERR_ARG, "dt must be finite and nonzero");
+        const double x0 = xmin ? xmin[l] : -1.0, x1 = xmax ? xmax[l] : 1.0;
+        if (!(x1 > x0)) return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin.");
+        b->h_off[l] = off;
+        off += ((int64_t)ncells[l] + 31) / 32 * 32;   // 256-byte aligned line starts
+        b->h_n.push_back(ncells[l]);
+        b->h_k.push_back(k[l]);
+        b->h_dt.push_back(dt[l]);
+        b->h_xmin.push_back(x0);
+        b->h_rx.push_back((x1 - x0) / ncells[l]);
+    }
+    b->h_off[nlines] = off;
+    b->total = off;
+    std::vector<long long> offs(b->h_off.begin(), b->h_off.end());
+    CALL(upload(&b->d_off, offs));
+    CALL(upload(&b->d_n, b->h_n));
+    CALL(upload(&b->d_k, b->h_k));
+    CALL(upload(&b->d_h, b->h_dt));
+    CALL(upload(&b->d_xmin, b->h_xmin));
+    CALL(upload(&b->d_rx, b->h_rx));
+    CALL(dalloc(&b->ui, (size_t)off));
+    CALL(dalloc(&b->ui2, (size_t)off));
+    CALL(dalloc(&b->slots, (size_t)3 * nlines));
+    CALL(batch_chunks(b, b->R));
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->own_stream = true;
+    *out = guard.release();
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_destroy(hrw_batch_t b)
+{
+    delete b;
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_size(hrw_batch_t b, int64_t *total, int64_t *offsets)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (total) *total = b->total;
+    if (offsets) std::memcpy(offsets, b->h_off.data(), b->h_off.size() * sizeof(int64_t));
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_set_stream(hrw_batch_t b, hrw_stream_t stream)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (!stream) return fail(HRW_ERR_ARG, "the batch needs a non-default stream (graph capture)");
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    b->own_stream = false;
+    b->stream = S(stream);
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return HRW_OK;
+}
+
+template <int FLUX, bool AMAX, int R>
+static void lbatch(const BatchArgs &a, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((a.nchunks * 64 + kBlock - 1) / kBlock);
+    batch_stage_kernel<FLUX, AMAX, R><<<grid, kBlock, 0, s>>>(a);
+}
+
+static int batch_stage(hrw_batch_s *b, int seq, int stage, const double *us, const double *u0,
+                       double *uo)
+{
+    BatchArgs a{};
+    a.us = us;
+    a.u0 = u0;
+    a.uo = uo;
+    a.off = b->d_off;
+    a.n = b->d_n;
+    a.k = b->d_k;
+    a.h = b->d_h;
+    a.xmin = b->d_xmin;
+    a.rx = b->d_rx;
+    a.chunk_line = b->d_cline;
+    a.chunk_start = b->d_cstart;
+    a.nchunks = b->nchunks;
+    a.nlines = b->nlines;
+    a.stage = stage;
+    a.eps = b->eps;
+    a.alpha = b->alpha;
+    const bool amax = b->alphamax();
+    if (amax) {
+        a.alpha_in = b->slots + (size_t)(seq % 3) * b->nlines;
+        a.alpha_out = b->slots + (size_t)((seq + 1) % 3) * b->nlines;
+        a.alpha_clear = b->slots + (size_t)((seq + 2) % 3) * b->nlines;
+    }
+    const bool lf = b->numflux == HRW_FLUX_LAX_FRIEDRICHS;
+    if (b->chunk_R == 4) {
+        if (lf && amax) lbatch<LF, true, 4>(a, b->stream);
+        else if (lf) lbatch<LF, false, 4>(a, b->stream);
+        else lbatch<GODUNOV, false, 4>(a, b->stream);
+    } else {
+        if (lf && amax) lbatch<LF, true, 8>(a, b->stream);
+        else if (lf) lbatch<LF, false, 8>(a, b->stream);
+        else lbatch<GODUNOV, false, 8>(a, b->stream);
+    }
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+static int batch_seed(hrw_batch_s *b, const double *u)
+{
+    if (!b->alphamax()) return HRW_OK;
+    fill_u64_kernel<<<grid_for(3LL * b->nlines, 256, 4096), 256, 0, b->stream>>>(b->slots, 3LL * b->nlines);
+    batch_maxabs_kernel<<<(unsigned)((b->nchunks * 64 + kBlock - 1) / kBlock), kBlock, 0, b->stream>>>(
+        u, b->d_off, b->d_n, b->d_cline, b->d_cstart, b->nchunks, b->chunk_R, b->slots);
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+// The three RK3 stages of one step for every line (src/hrweno_tvdode.f90:162-167).
+static int batch_rk3_step(hrw_batch_s *b, double *u)
+{
+    CALL(batch_stage(b, 0, RK3_1, u, u, b->ui));
+    CALL(batch_stage(b, 1, RK3_2, b->ui, u, b->ui2));
+    CALL(batch_stage(b, 2, RK3_3, b->ui2, u, u));
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_step(hrw_batch_t b, double *u, int64_t nsteps, double *t, int use_graph)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (!u) return fail(HRW_ERR_ARG, "null state");
+    if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
+    if (((uintptr_t)u & 15u) != 0) return fail(HRW_ERR_ARG, "the batch state must be 16-byte aligned");
+    if (nsteps == 0) return HRW_OK;
+    CALL(batch_seed(b, u));
+    if (use_graph) {
+        if (!b->graph || b->graph_u != u || b->graph_R != b->chunk_R) {
+            if (b->graph) {
+                (void)hipGraphExecDestroy(b->graph);
+                b->graph = nullptr;
+            }
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int rc = batch_rk3_step(b, u);
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc != HRW_OK) return rc;
+            if (e != hipSuccess) return fail(HRW_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(HRW_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            b->graph_u = u;
+            b->graph_R = b->chunk_R;
+        }
+        for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+    } else {
+        for (int64_t s = 0; s < nsteps; ++s) CALL(batch_rk3_step(b, u));
+    }
+    if (t)
+        for (int l = 0; l < b->nlines; ++l)
+            for (int64_t s = 0; s < nsteps; ++s) t[l] = t[l] + b->h_dt[l];
+    return HRW_OK;
+}
+
+// The plain path: every line advanced by its own launches of the single-line stage kernel.
+extern "C" int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps, double *t)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (!u) return fail(HRW_ERR_ARG, "null state");
+    if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
+    const size_t per = (size_t)3 * kNSlot * kSlotStride;
+    if (!b->line_slots) CALL(dalloc(&b->line_slots, per * b->nlines));
+    hrw_problem_s p;
+    p.kind = PROB_BURGERS1D;
+    p.eps = b->eps;
+    p.numflux = b->numflux;
+    p.alpha = b->alpha;
+    p.arith = HRW_ARITH_FAST;
+    p.bc = HRW_BC_PERIODIC;
+    p.cells_per_lane = b->chunk_R;
+    for (int l = 0; l < b->nlines; ++l) {
+        p.neq = b->h_n[l];
+        p.k = b->h_k[l];
+        p.xmin = b->h_xmin[l];
+        p.rx = b->h_rx[l];
+        unsigned long long *sl = b->line_slots + per * l;
+        double *ul = u + b->h_off[l], *a1 = b->ui + b->h_off[l], *a2 = b->ui2 + b->h_off[l];
+        CALL(seed_alpha(&p, sl, ul, b->stream));
+        int seq = 0;
+        for (int64_t s = 0; s < nsteps; ++s) {
+            StageIO io;
+            io.us = ul; io.u0 = ul; io.uo = a1;
+            CALL(run_stage(&p, sl, seq++, RK3_1, b->h_dt[l], io, b->stream));
+            io.us = a1; io.u0 = ul; io.uo = a2;
+            CALL(run_stage(&p, sl, seq++, RK3_2, b->h_dt[l], io, b->stream));
+            io.us = a2; io.u0 = ul; io.uo = ul;
+            CALL(run_stage(&p, sl, seq++, RK3_3, b->h_dt[l], io, b->stream));
+        }
+        if (t)
+            for (int64_t s = 0; s < nsteps; ++s) t[l] = t[l] + b->h_dt[l];
+    }
+    return HRW_OK;
+}
+
+// Autotuner: times the batched RK3 step for each candidate chunk width on a scratch copy of
+// the state and keeps the fastest (the result is the same for every candidate).
+extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    double *scratch = nullptr;
+    CALL(dalloc(&scratch, (size_t)b->total));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int best = b->chunk_R;
+    float best_ms = 1e30f;
+    for (int R : {4, 8}) {
+        int rc = batch_chunks(b, R);
+        if (rc == HRW_OK) {
+            (void)hipMemcpyAsync(scratch, u, b->total * sizeof(double), hipMemcpyDeviceToDevice, b->stream);
+            rc = batch_seed(b, scratch);
+            if (rc == HRW_OK) rc = batch_rk3_step(b, scratch);   // warm-up
+            (void)hipEventRecord(e0, b->stream);
+            for (int i = 0; rc == HRW_OK && i < 3; ++i) rc = batch_rk3_step(b, scratch);
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rc != HRW_OK) {
+            dfree(scratch);
+            return rc;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = R;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(scratch);
+    b->R = best;
+    CALL(batch_chunks(b, best));
+    if (chosen) *chosen = best;
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (cells_per_lane != 4 && cells_per_lane != 8) return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
+    b->R = cells_per_lane;
+    return batch_chunks(b, cells_per_lane);
 }

@@ -138,23 +138,94 @@ template <int R> __device__ __forceinline__ void store_run(double *p, const doub
     }
 }
 
-// Divergences L of the lane's R cells from v[i] <-> cell c0-H+i.
-template <int K, int FLUX, bool PERIODIC, int R>
-__device__ __forceinline__ void lane_divergence(const Stage1DArgs &a, const double *v, int c0,
-                                                int n, bool interior, double alpha, double *Lv)
+// Stage combination in one form for every TVD stage: with L = rhs(us),
+//   un = post * (C*L + B*p1 + A*us + D*p2)      (post = 1/3 uses div3_fast)
+// evaluated as fma(D, p2, fma(C, L, fma(B, p1, A*us))).  p1/p2 are the pointwise operands
+// (u at the start of the step; or u_{n-4}, L(u_{n-4}) for the multistep method).
+struct StageCoef {
+    double A, B, C, D, post;
+    bool div3;
+};
+
+__host__ __device__ inline StageCoef stage_coef(int stage, double h)
 {
+    switch (stage) {
+    case RK1:
+    case RK2_1:
+    case RK3_1: return {1.0, 0.0, h, 0.0, 1.0, false};          // us + h L
+    case RK2_2: return {1.0, 1.0, h, 0.0, 0.5, false};          // (u0 + us + h L)/2
+    case RK3_2: return {1.0, 3.0, h, 0.0, 0.25, false};         // (3u0 + us + h L)/4
+    case RK3_3: return {2.0, 1.0, 2.0 * h, 0.0, 1.0, true};     // (u0 + 2us + 2h L)/3
+    case MS3: return {25.0, 7.0, 50.0 * h, 10.0 * h, 0.03125, false};
+    default: return {1.0, 0.0, 0.0, 0.0, 1.0, false};           // rhs only: un = us
+    }
+}
+
+// One wavefront's chunk [W, W + 64R) of a line: lane l owns the R cells from c0 = W + l*R.
+// Halo: H = K cells on each side (K-1 for the reconstruction window, one more for the
+// neighbouring reconstruction that the chunk-edge interface needs).  Interior chunks (range and
+// halo inside the line) take vector loads/stores and no boundary checks.  NPW = number of
+// pointwise operands of the stage (0, 1 or 2).  Returns the lane's max|output| (the next
+// stage's alpha contribution).  Requires n < 2^31 and 16-byte aligned arrays (checked on the
+// host).  Must be called by all 64 lanes of the wavefront.
+template <int K, int FLUX, bool PERIODIC, int R, int NPW>
+__device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha)
+{
+    constexpr int H = K;
+    constexpr int NV = R + 2 * H;
+    const int lane = threadIdx.x & 63;
+    const int n = (int)a.n;
+    const int c0 = W + lane * R;
+    const StageCoef sc = stage_coef(a.stage, a.h);
+    const bool write_u = a.stage >= 0;
+
+    // ---- the lane's cells and halo (v[i] <-> cell c0 - H + i) and pointwise operands
+    double v[NV];
+    [[maybe_unused]] double p1[NPW >= 1 ? R : 1], p2[NPW >= 2 ? R : 1];
+    const double *s1 = a.stage == MS3 ? a.uo4 : a.u0;
+    const bool interior = W >= H && (long long)W + 64LL * R + H <= n;
+    if (interior) {
+        load_run<R>(a.us + c0, v + H);
+        if constexpr (NPW >= 1) load_run<R>(s1 + c0, p1);
+        if constexpr (NPW >= 2) load_run<R>(a.Lo4 + c0, p2);
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            v[i] = __shfl_up(v[R + i], 1, 64);            // lane-1's last H cells
+            v[H + R + i] = __shfl_down(v[H + i], 1, 64);  // lane+1's first H cells
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) v[i] = a.us[c0 - H + i];
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) v[H + R + i] = a.us[c0 + R + i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = a.us[cell_index(c0 - H + i, n, PERIODIC)];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int g = c0 + i < n ? c0 + i : n - 1;
+            if constexpr (NPW >= 1) p1[i] = s1[g];
+            if constexpr (NPW >= 2) p2[i] = a.Lo4[g];
+        }
+    }
+
+    // ---- march over the cells: reconstruction of cell c0-1+s for s = 0..R+1
+    double m_out = 0.0;
     double fl = 0.0, vr_prev = 0.0;
     double e_prev = linear_edge32(a.xmin, a.rx, c0 < n ? c0 : n);
     const double eps12 = 12.0 * a.eps;
+    double un_pair[2], L_pair[2];
     [[maybe_unused]] Tri3 t2, t1;
-    [[maybe_unused]] Pair2 p1;
+    [[maybe_unused]] Pair2 pp1;
     if constexpr (K == 3) {
         t2 = tri3(v[0], v[1], v[2]);
         t1 = tri3(v[1], v[2], v[3]);
     } else if constexpr (K == 2) {
-        p1 = pair2(v[0], v[1]);
+        pp1 = pair2(v[0], v[1]);
     }
-    // reconstruction of cell c0-1+s for s = 0..R+1
 #pragma unroll
     for (int s = 0; s <= R + 1; ++s) {
         double vl, vr;
@@ -165,8 +236,8 @@ __device__ __forceinline__ void lane_divergence(const Stage1DArgs &a, const doub
             t1 = t0;
         } else if constexpr (K == 2) {
             const Pair2 p0 = pair2(v[s + 1], v[s + 2]);
-            recon2(a.eps, p1, p0, vl, vr);
-            p1 = p0;
+            recon2(a.eps, pp1, p0, vl, vr);
+            pp1 = p0;
         } else {
             vl = vr = v[s];
         }
@@ -181,7 +252,7 @@ __device__ __forceinline__ void lane_divergence(const Stage1DArgs &a, const doub
             fl = f;
             continue;
         }
-        // cell g = c0 + s - 2 between fluxes fleft and fr
+        // cell g = c0 + i, i = s - 2, between fluxes fleft and fr
         const int i = s - 2;
         const int g = c0 + i;
         double fr = f, fleft = fl;
@@ -200,127 +271,146 @@ __device__ __forceinline__ void lane_divergence(const Stage1DArgs &a, const doub
                 if (g == n - 1) fr = fleft;
             }
         }
-        Lv[i] = (fleft - fr) * rcp_nr(w);
+        const double L = (fleft - fr) * rcp_nr(w);
+        const double us = v[H + i];
+        double t = sc.A * us;
+        if constexpr (NPW >= 1) t = fma_(sc.B, p1[i], t);
+        t = fma_(sc.C, L, t);
+        if constexpr (NPW >= 2) t = fma_(sc.D, p2[i], t);
+        const double un = sc.div3 ? div3_fast(t) : sc.post * t;
+        // write back in pairs (16-byte stores) inside the line, one by one at its ends
+        un_pair[i & 1] = un;
+        L_pair[i & 1] = L;
+        if (interior) {
+            m_out = fmax(m_out, fabs(un));
+            if (i & 1) {
+                if (write_u) *reinterpret_cast<double2 *>(a.uo + g - 1) = make_double2(un_pair[0], un_pair[1]);
+                if (a.Lout) *reinterpret_cast<double2 *>(a.Lout + g - 1) = make_double2(L_pair[0], L_pair[1]);
+            }
+        } else if (g < n) {
+            m_out = fmax(m_out, fabs(un));
+            if (write_u) a.uo[g] = un;
+            if (a.Lout) a.Lout[g] = L;
+        }
     }
+    return m_out;
 }
 
-// Wave w of block b owns cells [W, W + 64R), W = (4b + w)*64R; lane l owns R cells from
-// c0 = W + l*R.  Halo: H = K cells on each side (K-1 for the reconstruction window, one more
-// for the neighbouring reconstruction that the edge interface needs).  Interior waves (the
-// whole range and halo inside the line) take vector loads/stores and no boundary checks.
-// Requires n < 2^31 and 16-byte aligned arrays (checked on the host).
+template <int K, int FLUX, bool PERIODIC, int R>
+__device__ __forceinline__ double chunk_stage_any(const Stage1DArgs &a, int W, double alpha)
+{
+    if (a.stage == MS3) return chunk_stage<K, FLUX, PERIODIC, R, 2>(a, W, alpha);
+    if (a.stage == RK2_2 || a.stage == RK3_2 || a.stage == RK3_3)
+        return chunk_stage<K, FLUX, PERIODIC, R, 1>(a, W, alpha);
+    return chunk_stage<K, FLUX, PERIODIC, R, 0>(a, W, alpha);
+}
+
 #ifndef HRW_FAST_WAVES_PER_EU
 #define HRW_FAST_WAVES_PER_EU 4
 #endif
+
+// One line: wave w of block b owns the chunk starting at W = (4b + w)*64R.
 template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_FAST_WAVES_PER_EU)))
 void stage1d_fast_kernel(Stage1DArgs a)
 {
-    constexpr int H = K;
-    constexpr int NV = R + 2 * H;
     __shared__ double s_red[kBlock / 64 + 1];
     const int lane = threadIdx.x & 63;
-    const int n = (int)a.n;
     const int W = (int)(((long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64LL * R));
-    const int c0 = W + lane * R;
-
     double alpha = a.alpha;
     if constexpr (ALPHAMAX) {
         // every wave reduces the 64 partial maxima itself (no barrier)
         alpha = wave_max(__longlong_as_double((long long)a.alpha_in[lane * kSlotStride]));
     }
     clear_set(a.alpha_clear);
-
-    // ---- phase 1: the lane's cells and halo (v[i] <-> cell c0 - H + i), then L
-    double v[NV];
-    const bool interior = W >= H && (long long)W + 64LL * R + H <= n;
-    if (interior) {
-        load_run<R>(a.us + c0, v + H);
-#pragma unroll
-        for (int i = 0; i < H; ++i) {
-            v[i] = __shfl_up(v[R + i], 1, 64);            // lane-1's last H cells
-            v[H + R + i] = __shfl_down(v[H + i], 1, 64);  // lane+1's first H cells
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < H; ++i) v[i] = a.us[c0 - H + i];
-        }
-        if (lane == 63) {
-#pragma unroll
-            for (int i = 0; i < H; ++i) v[H + R + i] = a.us[c0 + R + i];
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] = a.us[cell_index(c0 - H + i, n, PERIODIC)];
-    }
-    double Lv[R];
-    lane_divergence<K, FLUX, PERIODIC, R>(a, v, c0, n, interior, alpha, Lv);
-
-    // ---- phase 2: stage combination with the pointwise operands, then write back
-    double un[R];
-    const double *us = v + H;
-    const int stage = a.stage;
-    const double h = a.h;
-    double pw[R], pw2[R];
-    auto load_pw = [&](const double *src, double *dst) {
-        if (interior) {
-            load_run<R>(src + c0, dst);
-        } else {
-#pragma unroll
-            for (int i = 0; i < R; ++i) dst[i] = src[c0 + i < n ? c0 + i : n - 1];
-        }
-    };
-    switch (stage) {
-    case RK1:
-    case RK2_1:
-    case RK3_1:
-#pragma unroll
-        for (int i = 0; i < R; ++i) un[i] = fma_(h, Lv[i], us[i]);
-        break;
-    case RK2_2:
-        load_pw(a.u0, pw);
-#pragma unroll
-        for (int i = 0; i < R; ++i) un[i] = 0.5 * fma_(h, Lv[i], pw[i] + us[i]);
-        break;
-    case RK3_2:
-        load_pw(a.u0, pw);
-#pragma unroll
-        for (int i = 0; i < R; ++i) un[i] = 0.25 * fma_(h, Lv[i], fma_(3.0, pw[i], us[i]));
-        break;
-    case RK3_3:
-        load_pw(a.u0, pw);
-#pragma unroll
-        for (int i = 0; i < R; ++i) un[i] = div3_fast(fma_(2.0 * h, Lv[i], fma_(2.0, us[i], pw[i])));
-        break;
-    case MS3:
-        load_pw(a.uo4, pw);
-        load_pw(a.Lo4, pw2);
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-            un[i] = 0.03125 * fma_(10.0 * h, pw2[i], fma_(7.0, pw[i], fma_(50.0 * h, Lv[i], 25.0 * us[i])));
-        break;
-    default:   // rhs only: the next launch's alpha is max|us|
-#pragma unroll
-        for (int i = 0; i < R; ++i) un[i] = us[i];
-        break;
-    }
-    double m_out = 0.0;
-    if (interior) {
-#pragma unroll
-        for (int i = 0; i < R; ++i) m_out = fmax(m_out, fabs(un[i]));
-        if (stage >= 0) store_run<R>(a.uo + c0, un);
-        if (a.Lout) store_run<R>(a.Lout + c0, Lv);
-    } else {
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            const int g = c0 + i;
-            if (g >= n) continue;   // cells past the end are not part of the state
-            m_out = fmax(m_out, fabs(un[i]));
-            if (stage >= 0) a.uo[g] = un[i];
-            if (a.Lout) a.Lout[g] = Lv[i];
-        }
-    }
+    const double m_out = chunk_stage_any<K, FLUX, PERIODIC, R>(a, W, alpha);
     if constexpr (ALPHAMAX) publish_max(m_out, a.alpha_out, s_red);
+}
+
+// --------------------------------------------------------------------------------------
+// Many independent periodic lines in one launch.  Line l: cells [off[l], off[l] + n[l]) of
+// the flat state, WENO order k[l], step h[l], linear grid xmin[l] + rx[l]*i.  Chunk c of the
+// table covers cells [start[c], start[c] + 64R) of line line[c]; wavefront c of the grid takes
+// chunk c.  Lax-Friedrichs alpha = max|v| per line: slot set s holds one partial per line.
+struct BatchArgs {
+    const double *us, *u0;
+    double *uo;
+    const long long *off;
+    const int *n, *k;
+    const double *h, *xmin, *rx;
+    const int *chunk_line, *chunk_start;
+    long long nchunks;
+    int nlines, stage, flux;
+    double eps, alpha;
+    const unsigned long long *alpha_in;   // nlines partials, read
+    unsigned long long *alpha_out;        // nlines partials, atomicMax
+    unsigned long long *alpha_clear;      // nlines partials, cleared
+};
+
+template <int FLUX, bool ALPHAMAX, int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_FAST_WAVES_PER_EU)))
+void batch_stage_kernel(BatchArgs b)
+{
+    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (ALPHAMAX && b.alpha_clear) {
+        for (long long l = (long long)blockIdx.x * kBlock + threadIdx.x; l < b.nlines;
+             l += (long long)gridDim.x * kBlock)
+            b.alpha_clear[l] = 0ull;
+    }
+    if (wave >= b.nchunks) return;   // whole wavefronts leave together
+    const int line = b.chunk_line[wave];
+    const long long off = b.off[line];
+    Stage1DArgs a{};
+    a.us = b.us + off;
+    a.u0 = b.u0 + off;
+    a.uo = b.uo + off;
+    a.n = b.n[line];
+    a.xmin = b.xmin[line];
+    a.rx = b.rx[line];
+    a.eps = b.eps;
+    a.h = b.h[line];
+    a.stage = b.stage;
+    double alpha = b.alpha;
+    if constexpr (ALPHAMAX) alpha = __longlong_as_double((long long)b.alpha_in[line]);
+    const int W = b.chunk_start[wave];
+    double m;
+    switch (b.k[line]) {
+    case 1: m = chunk_stage_any<1, FLUX, true, R>(a, W, alpha); break;
+    case 2: m = chunk_stage_any<2, FLUX, true, R>(a, W, alpha); break;
+    default: m = chunk_stage_any<3, FLUX, true, R>(a, W, alpha); break;
+    }
+    if constexpr (ALPHAMAX) {
+        m = wave_max(m);
+        if (lane == 0) atomicMax(b.alpha_out + line, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// max|u| of every line into slot set `set` (one partial per line).
+__global__ __launch_bounds__(kBlock) void batch_maxabs_kernel(const double *__restrict__ u,
+                                                              const long long *off, const int *n,
+                                                              const int *chunk_line,
+                                                              const int *chunk_start,
+                                                              long long nchunks, int R,
+                                                              unsigned long long *set)
+{
+    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= nchunks) return;
+    const int line = chunk_line[wave];
+    const long long base = off[line] + chunk_start[wave];
+    const int len = min(64 * R, n[line] - chunk_start[wave]);
+    double m = 0.0;
+    for (int i = lane; i < len; i += 64) m = fmax(m, fabs(u[base + i]));
+    m = wave_max(m);
+    if (lane == 0) atomicMax(set + line, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, long long count)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0ull;
 }
 
 }  // namespace hrw

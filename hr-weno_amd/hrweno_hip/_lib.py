@@ -69,6 +69,14 @@ SIGNATURES = {
     "hrw_ode_set_stream": [P, P],
     "hrw_ode_integrate": [P, P, PD, D, D, I],
     "hrw_ode_info": [P, PL, PI, PI, PI],
+    "hrw_batch_burgers_create": [L, PI, PI, PD, PD, PD, D, I, D, I, C.POINTER(P)],
+    "hrw_batch_destroy": [P],
+    "hrw_batch_size": [P, PL, PL],
+    "hrw_batch_set_stream": [P, P],
+    "hrw_batch_step": [P, P, L, PD, I],
+    "hrw_batch_step_per_line": [P, P, L, PD],
+    "hrw_batch_tune": [P, P, PI],
+    "hrw_batch_set_cells_per_lane": [P, I],
 }
 
 _lib = None

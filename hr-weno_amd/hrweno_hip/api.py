@@ -337,6 +337,68 @@ class MsTvd(_Ode):
         self._h = h
 
 
+class BurgersBatch:
+    """Ensemble of independent periodic 1D Burgers lines of uneven length, each with its own
+    WENO order k, grid and time step, advanced together (one launch per RK3 stage for all
+    lines, optionally from a captured hipGraph).  The state is one flat device tensor; line l
+    is ``u[offsets[l]:offsets[l] + ncells[l]]``."""
+
+    def __init__(self, ncells, k, dt, xmin=None, xmax=None, eps=1e-6, flux=LAX_FRIEDRICHS,
+                 alpha=-1.0):
+        n = np.ascontiguousarray(ncells, dtype=np.int32)
+        kk = np.ascontiguousarray(k, dtype=np.int32)
+        d = _np(dt)
+        x0 = _np(xmin) if xmin is not None else None
+        x1 = _np(xmax) if xmax is not None else None
+        h = C.c_void_p()
+        call("hrw_batch_burgers_create", n.size, n.ctypes.data_as(_lib.PI),
+             kk.ctypes.data_as(_lib.PI), _pd(d), _pd(x0) if x0 is not None else None,
+             _pd(x1) if x1 is not None else None, float(eps), int(flux), float(alpha), FAST,
+             C.byref(h))
+        self._h = h
+        self.ncells, self.k, self.dt = n, kk, d
+        total = C.c_int64()
+        offs = np.zeros(n.size + 1, dtype=np.int64)
+        call("hrw_batch_size", h, C.byref(total), offs.ctypes.data_as(_lib.PL))
+        self.total, self.offsets = total.value, offs
+        self.t = np.zeros(n.size)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().hrw_batch_destroy(self._h)
+            self._h = None
+
+    def pack(self, lines, device):
+        """Flat device state from a list of per-line host arrays."""
+        torch = _torch()
+        flat = np.zeros(self.total)
+        for l, a in enumerate(lines):
+            flat[self.offsets[l]:self.offsets[l] + self.ncells[l]] = a
+        return torch.as_tensor(flat, device=device).clone()
+
+    def unpack(self, u):
+        flat = u.cpu().numpy()
+        return [flat[self.offsets[l]:self.offsets[l] + self.ncells[l]].copy()
+                for l in range(self.ncells.size)]
+
+    def set_stream(self, stream):
+        call("hrw_batch_set_stream", self._h, C.c_void_p(stream))
+
+    def tune(self, u) -> int:
+        r = C.c_int()
+        call("hrw_batch_tune", self._h, _dptr(u), C.byref(r))
+        return r.value
+
+    def set_cells_per_lane(self, r: int):
+        call("hrw_batch_set_cells_per_lane", self._h, int(r))
+
+    def step(self, u, nsteps: int, graph: bool = True):
+        call("hrw_batch_step", self._h, _dptr(u), int(nsteps), _pd(self.t), int(bool(graph)))
+
+    def step_per_line(self, u, nsteps: int):
+        call("hrw_batch_step_per_line", self._h, _dptr(u), int(nsteps), _pd(self.t))
+
+
 def is_done(t: float, tout: float, dt: float) -> bool:
     """src/hrweno_tvdode.f90:273-284."""
     return (t - tout) * np.copysign(1.0, dt) > 0.0

@@ -184,6 +184,29 @@ int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tout, double d
 /* Public fields neq, order, fevals, istate (src/hrweno_tvdode.f90:18-29). */
 int hrw_ode_info(hrw_ode_t ode, int64_t *neq, int *order, int *fevals, int *istate);
 
+/* ---------------------------------------------------------------- batched lines ---- */
+/* Ensemble of nlines independent periodic 1D Burgers lines (the many-lines use of
+ * rktvd(fu, neq, 3)%integrate, src/hrweno_tvdode.f90:160-171, with the example-1 RHS):
+ * line l has ncells[l] cells on [xmin[l], xmax[l]] (NULL: [-1, 1]), WENO order k[l] and its own
+ * time step dt[l].  FAST arithmetic only.  The state is one device array of `total` doubles;
+ * line l occupies [offsets[l], offsets[l] + ncells[l]) (offsets are 256-byte aligned). */
+int hrw_batch_burgers_create(int64_t nlines, const int *ncells, const int *k, const double *dt,
+                             const double *xmin, const double *xmax, double eps, int numflux,
+                             double alpha, int arith, hrw_batch_t *out);
+int hrw_batch_destroy(hrw_batch_t b);
+/* total doubles of the state; offsets (nlines+1 values, may be NULL). */
+int hrw_batch_size(hrw_batch_t b, int64_t *total, int64_t *offsets);
+int hrw_batch_set_stream(hrw_batch_t b, hrw_stream_t stream);
+/* nsteps SSP-RK3 steps of every line: one launch per stage for all lines; use_graph != 0 replays
+ * a captured hipGraph of one step.  t (host, nlines, may be NULL) advances by nsteps*dt[l]. */
+int hrw_batch_step(hrw_batch_t b, double *u, int64_t nsteps, double *t, int use_graph);
+/* The same, each line advanced by its own launches (the one-launch-per-line path). */
+int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps, double *t);
+/* Autotuner: times each chunk width (cells per lane 4 or 8) on a scratch copy of u and keeps
+ * the fastest; or set it directly. */
+int hrw_batch_tune(hrw_batch_t b, const double *u, int *cells_per_lane);
+int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane);
+
 #ifdef __cplusplus
 }
 #endif

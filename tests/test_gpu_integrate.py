@@ -157,6 +157,23 @@ def test_mstvd_burgers_exact(oracle, hh, dev):
     assert np.array_equal(u, ou)
 
 
+@pytest.mark.parametrize("r", [4, 8])
+def test_fast_mstvd_and_rhs(oracle, hh, dev, r):
+    """FAST multistep (two pointwise operands per cell) and rhs-only passes of the 1D kernel."""
+    from hrweno_hip import workloads as W
+    c = W.config1(n=3000, seed=12)
+    p = hh.Burgers1D(c.n, 3, 1e-6, arith="fast").tune(0, r)
+    ode = hh.MsTvd(p)
+    u = _t(c.u0, dev)
+    t = ode.integrate(u, 0.0, 0.05, c.dt)
+    ou, ot, fe = oracle.burgers_ms(c.u0, c.width, tout=0.05, dt=c.dt)
+    assert t == ot and ode.fevals == fe
+    assert_close(_h(u), ou)
+    L = _h(p.rhs(0.0, _t(c.u0, dev)))
+    ol, *_ = oracle.burgers_rk(c.u0, c.width, order=1, tout=0.5, dt=1.0, itask=2)
+    assert_close(c.u0 + L, ol)
+
+
 def test_mstvd_resume_across_calls(oracle, hh, dev):
     """History persists across integrate calls (istate 2), as in the reference."""
     from hrweno_hip import workloads as W
