@@ -200,12 +200,101 @@ def run_pbe(args, torch, hh, W, dev):
         "synthetic (seeded Gaussian pulse)"
 
 
+def run_adjoint(args, torch, hh, W, dev):
+    """Gradient of 0.5*|u_N - target|^2 w.r.t. u0 for 2**20 cells x 400 RK3 steps, checkpoints
+    every 20 steps: the recording forward + the backward (recompute + reverse kernels)."""
+    c, noise = W.config5(steps=args.steps)
+    n, steps = c.n, c.steps
+    prob = hh.Burgers1D(n)
+    adj = hh.Adjoint(prob, nsteps=steps, dt=c.dt, ckpt_every=args.ckpt_every)
+    u0 = torch.as_tensor(c.u0, device=dev).clone()
+    target = adj.forward(torch.as_tensor(c.u0 + noise, device=dev)).clone()
+    res = {}
+    adj.backward(adj.forward(u0) - target)          # warm-up
+    ms_f = timed(torch, lambda: res.__setitem__("uN", adj.forward(u0)))
+    gN = res["uN"] - target
+    ms_b = timed(torch, lambda: res.__setitem__("g0", adj.backward(gN)))
+    if not bool(torch.isfinite(res["g0"]).all()):
+        raise RuntimeError("non-finite gradient")
+    # the plain forward of the same steps (no recording)
+    ode = hh.RkTvd(prob, order=3)
+    up = u0.clone()
+    ms_p = timed(torch, lambda: ode.integrate(up, 0.0, W.steps_tout(0.0, c.dt, steps), c.dt))
+    # the reverse kernels alone: every stage input kept by the forward, nothing recomputed
+    full = hh.Adjoint(prob, nsteps=steps, dt=c.dt, ckpt_every=steps)
+    full.forward(u0)
+    full.backward(gN)
+    ms_r = timed(torch, lambda: full.backward(gN))
+    info = adj.info()
+    log(f"adjoint: forward(recording) {ms_f:.2f} ms, backward {ms_b:.2f} ms, plain forward "
+        f"{ms_p:.2f} ms, reverse-only {ms_r:.2f} ms")
+    rl = roofline(88.0 / 3 * n, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<4,alphamax>",
+                  "adjoint_2p20")
+    config = {"workload": "burgers1d_weno5_lf_rk3_adjoint_2^20cells_400steps", "cells": n,
+              "steps": steps, "ckpt_every": info["ckpt_every"], "k": 3, "eps": 1e-6,
+              "loss": "0.5*|u_N - F(u0 + N(0,0.01))|^2", "parallelism": "single GPU",
+              "value_is": "cell-steps of the whole gradient (recording forward + backward) per s",
+              "forward_only_cell_updates_per_s": n * steps / (ms_p / 1e3),
+              "gradient_over_forward": (ms_f + ms_b) / ms_p,
+              "backward_over_forward": ms_b / ms_p,
+              "reverse_only_over_forward": ms_r / ms_p,
+              "device_bytes": info["device_bytes"]}
+    return n * steps / ((ms_f + ms_b) / 1e3), (ms_f + ms_b) / steps, rl, config, \
+        "synthetic (seeded SplitMix64, config-1 style data; target from N(0,0.01) noise)"
+
+
+def run_adjoint_batch(args, torch, hh, W, dev):
+    """The batched adjoint: 256 lines of 1024-16384 cells, every stage of every line in one
+    launch, vs one adjoint object per line."""
+    import numpy as np
+    cases, noises = W.config5_batch(steps=args.steps)
+    steps = args.steps
+    lens = np.array([c.n for c in cases])
+    b = hh.BurgersBatch(lens, np.full(lens.size, 3), np.array([c.dt for c in cases]))
+    adj = hh.Adjoint(b, nsteps=steps, ckpt_every=args.ckpt_every)
+    u0 = b.pack([c.u0 for c in cases], dev)
+    target = adj.forward(b.pack([c.u0 + z for c, z in zip(cases, noises)], dev)).clone()
+    res = {}
+    adj.backward(adj.forward(u0) - target)
+    ms = timed(torch, lambda: res.__setitem__("g", adj.backward(adj.forward(u0) - target)))
+    total = int(lens.sum())
+    pl_steps = max(1, min(steps, args.per_line_steps))
+    per = [hh.Adjoint(hh.Burgers1D(int(c.n)), nsteps=pl_steps, dt=c.dt,
+                      ckpt_every=args.ckpt_every) for c in cases]
+    lines = [torch.as_tensor(c.u0, device=dev) for c in cases]
+
+    def per_line():
+        for a, x in zip(per, lines):
+            a.backward(a.forward(x))
+    per_line()
+    ms_pl = timed(torch, per_line)
+    value = total * steps / (ms / 1e3)
+    pl_value = total * pl_steps / (ms_pl / 1e3)
+    log(f"batched adjoint: {steps} steps {ms:.2f} ms; per-line {pl_steps} steps {ms_pl:.2f} ms")
+    full = hh.Adjoint(b, nsteps=steps, ckpt_every=steps)   # reverse kernels alone
+    gN = adj.forward(u0) - target
+    full.forward(u0)
+    full.backward(gN)
+    ms_r = timed(torch, lambda: full.backward(gN))
+    rl = roofline(88.0 / 3 * total, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<4,alphamax>",
+                  "adjoint_batch")
+    config = {"workload": "burgers1d_batch_adjoint_256_lines_log_uniform_1024_16384",
+              "lines": len(cases), "cells": total, "steps": steps, "k": 3,
+              "ckpt_every": args.ckpt_every, "parallelism": "single GPU",
+              "value_is": "cell-steps of the whole gradient (recording forward + backward) per s",
+              "per_line_cell_updates_per_s": pl_value, "per_line_steps": pl_steps,
+              "speedup_vs_per_line": value / pl_value}
+    return value, ms / steps, rl, config, "synthetic (seeded SplitMix64, config-1 style lines)"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["plain", "batch", "pbe"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batch", "pbe", "adjoint", "adjoint_batch"],
+                    default="plain")
+    ap.add_argument("--ckpt-every", type=int, default=20)
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cells", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -223,7 +312,8 @@ def main() -> int:
               "by domain decomposition, which this build does not launch.", file=sys.stderr)
         return 2
     if args.steps is None:
-        args.steps = {"plain": 100, "batch": 200, "pbe": 500}[args.workload]
+        args.steps = {"plain": 100, "batch": 200, "pbe": 500, "adjoint": 400,
+                      "adjoint_batch": 400}[args.workload]
 
     import torch
     import hrweno_hip as hh
@@ -239,7 +329,8 @@ def main() -> int:
         log(f"copy probe: 20 copies of {args.cells * 8} bytes")
         return 0
 
-    runner = {"plain": run_plain, "batch": run_batch, "pbe": run_pbe}[args.workload]
+    runner = {"plain": run_plain, "batch": run_batch, "pbe": run_pbe, "adjoint": run_adjoint,
+              "adjoint_batch": run_adjoint_batch}[args.workload]
     value, ms_step, rl, config, data = runner(args, torch, hh, W, dev)
 
     cpu = None

@@ -21,6 +21,7 @@
 #include "kernels_1d.hpp"
 #include "kernels_1d_fast.hpp"
 #include "kernels_2d.hpp"
+#include "kernels_adjoint.hpp"
 
 using namespace hrw;
 
@@ -819,8 +820,22 @@ struct StageIO {
     const double *us = nullptr, *u0 = nullptr, *uo4 = nullptr, *Lo4 = nullptr;
     double *uo = nullptr, *Lout = nullptr;
 };
+// The same with the three alpha slot sets given explicitly (read / accumulate / clear).
+static int run_stage_sets(const hrw_problem_s *p, const unsigned long long *ain,
+                          unsigned long long *aout, unsigned long long *aclear, int stage,
+                          double h, const StageIO &io, hipStream_t s);
 static int run_stage(const hrw_problem_s *p, unsigned long long *slots, int seq, int stage,
                      double h, const StageIO &io, hipStream_t s)
+{
+    const size_t set = (size_t)kNSlot * kSlotStride;
+    if (p->alphamax())
+        return run_stage_sets(p, slots + (seq % 3) * set, slots + ((seq + 1) % 3) * set,
+                              slots + ((seq + 2) % 3) * set, stage, h, io, s);
+    return run_stage_sets(p, nullptr, nullptr, nullptr, stage, h, io, s);
+}
+static int run_stage_sets(const hrw_problem_s *p, const unsigned long long *ain,
+                          unsigned long long *aout, unsigned long long *aclear, int stage,
+                          double h, const StageIO &io, hipStream_t s)
 {
     if (p->kind == PROB_BURGERS1D) {
         Stage1DArgs a{};
@@ -839,10 +854,9 @@ static int run_stage(const hrw_problem_s *p, unsigned long long *slots, int seq,
         a.alpha = p->alpha;
         a.stage = stage;
         if (p->alphamax()) {
-            const size_t set = (size_t)kNSlot * kSlotStride;
-            a.alpha_in = slots + (seq % 3) * set;
-            a.alpha_out = slots + ((seq + 1) % 3) * set;
-            a.alpha_clear = slots + ((seq + 2) % 3) * set;
+            a.alpha_in = ain;
+            a.alpha_out = aout;
+            a.alpha_clear = aclear;
         }
         launch_stage1d(p, a, s);
     } else {
@@ -913,17 +927,22 @@ struct hrw_ode_s {
     double *ring_u[4] = {}, *ring_L[4] = {};
     int head = 0;                          // ring index of column 4 (oldest)
     unsigned long long *slots = nullptr;
-    ~hrw_ode_s()
-    {
-        dfree(ui);
-        dfree(ui2);
-        dfree(udot);
-        dfree(S);
-        for (auto &b : ring_u) dfree(b);
-        for (auto &b : ring_L) dfree(b);
-        dfree(slots);
-    }
+    int tape_every = 0;                    // > 0: integrate records for hrw_ode_adjoint
+    hrw_adjoint_s *adj = nullptr;
+    ~hrw_ode_s();
 };
+static int ode_free_adjoint(hrw_ode_s *o);
+hrw_ode_s::~hrw_ode_s()
+{
+    dfree(ui);
+    dfree(ui2);
+    dfree(udot);
+    dfree(S);
+    for (auto &b : ring_u) dfree(b);
+    for (auto &b : ring_L) dfree(b);
+    dfree(slots);
+    ode_free_adjoint(this);
+}
 
 static int ode_alloc(hrw_ode_s *o)
 {
@@ -1074,10 +1093,13 @@ static int rk_step(hrw_ode_s *o, int &seq, double t, double dt, double *u, doubl
     return HRW_OK;
 }
 
+static int rk_integrate_taped(hrw_ode_s *o, double *u, double *t, double tout, double dt,
+                              int itask);
 static int rk_integrate(hrw_ode_s *o, double *u, double *t, double tout, double dt, int itask)
 {
     if (o->istate < 1) return HRW_OK;                  // :126
     if (is_done(*t, tout, dt)) return HRW_OK;          // :127
+    if (o->tape_every > 0) return rk_integrate_taped(o, u, t, tout, dt, itask);
     int seq = 0;
     if (o->p) CALL(seed_alpha(o->p, o->slots, u, o->stream));
     double *cur = u, *other = o->ui2;
@@ -1313,8 +1335,20 @@ static void lbatch(const BatchArgs &a, hipStream_t s)
     batch_stage_kernel<FLUX, AMAX, R><<<grid, kBlock, 0, s>>>(a);
 }
 
+static int batch_stage_sets(hrw_batch_s *b, const unsigned long long *ain,
+                            unsigned long long *aout, unsigned long long *aclear, int stage,
+                            const double *us, const double *u0, double *uo);
 static int batch_stage(hrw_batch_s *b, int seq, int stage, const double *us, const double *u0,
                        double *uo)
+{
+    unsigned long long *sl = b->slots;
+    const size_t L = (size_t)b->nlines;
+    return batch_stage_sets(b, sl + (seq % 3) * L, sl + ((seq + 1) % 3) * L,
+                            sl + ((seq + 2) % 3) * L, stage, us, u0, uo);
+}
+static int batch_stage_sets(hrw_batch_s *b, const unsigned long long *ain,
+                            unsigned long long *aout, unsigned long long *aclear, int stage,
+                            const double *us, const double *u0, double *uo)
 {
     BatchArgs a{};
     a.us = us;
@@ -1335,9 +1369,9 @@ static int batch_stage(hrw_batch_s *b, int seq, int stage, const double *us, con
     a.alpha = b->alpha;
     const bool amax = b->alphamax();
     if (amax) {
-        a.alpha_in = b->slots + (size_t)(seq % 3) * b->nlines;
-        a.alpha_out = b->slots + (size_t)((seq + 1) % 3) * b->nlines;
-        a.alpha_clear = b->slots + (size_t)((seq + 2) % 3) * b->nlines;
+        a.alpha_in = ain;
+        a.alpha_out = aout;
+        a.alpha_clear = aclear;
     }
     const bool lf = b->numflux == HRW_FLUX_LAX_FRIEDRICHS;
     if (b->chunk_R == 4) {
@@ -1497,4 +1531,469 @@ extern "C" int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane)
     if (cells_per_lane != 4 && cells_per_lane != 8) return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
     b->R = cells_per_lane;
     return batch_chunks(b, cells_per_lane);
+}
+
+// ------------------------------------------------------------------ adjoint -----------
+// Discrete adjoint of nsteps fixed rktvd steps (src/hrweno_tvdode.f90:136-171) of periodic
+// Lax-Friedrichs Burgers: one line (from a problem) or every line of a batch.
+//   forward : u0 -> u_N, keeping a checkpoint every m steps; the stage inputs of the last
+//             segment and their alpha slot sets stay in the segment buffers;
+//   backward: dJ/du_N -> dJ/du_0, segment by segment from the last: the stage inputs of a
+//             segment are recomputed from its checkpoint (the same kernels, so the same
+//             values), then its steps are reversed stage by stage with adjoint_stage_kernel.
+// Segment s holds the stage inputs X[q], q = 0..order*len: X[0] = ckpt[s], X[q] = Y[q-1],
+// X[order*len] = ckpt[s+1]; launch q reads alpha set q, accumulates set q+1, clears set q+2,
+// so every set keeps max|X[q]| until the segment is recomputed.
+struct hrw_adjoint_s {
+    hrw_problem_s *p = nullptr;   // one line, or
+    hrw_batch_s *b = nullptr;     // every line of a batch (which must outlive this object)
+    int order = 3, m = 20, R = 4;
+    int64_t nsteps = 0, nseg = 0, flat = 0;
+    double dt = 0.0;
+    std::vector<double *> ckpt, Y;
+    double *lamA = nullptr, *lamB = nullptr;
+    unsigned long long *sets = nullptr;
+    size_t set_words = 0;
+    // reverse-pass tables (the batch's own line tables are borrowed)
+    long long *d_off = nullptr;
+    int *d_n = nullptr, *d_k = nullptr;
+    double *d_h = nullptr, *d_xmin = nullptr, *d_rx = nullptr;
+    bool own_lines = false;
+    int *d_cline = nullptr, *d_cstart = nullptr, *d_lchunk0 = nullptr, *d_lnch = nullptr;
+    long long nchunks = 0;
+    // alpha-bar bookkeeping of the reverse launches: launch r writes slot r % 4 (per-chunk
+    // partial sums, per-line argmax keys tagged with r's tag); the correction it owes to its
+    // output is applied by the launch that next reads that buffer (at most 3 launches later),
+    // or by a fixup at the end of the backward.
+    static constexpr int kSlots = 4;
+    double *partial[kSlots] = {};
+    unsigned long long *amax[kSlots] = {};
+    unsigned tag = 0;
+    int64_t launches = 0;
+    std::vector<std::pair<const double *, Pending>> owed;   // buffer -> its pending correction
+    int64_t y_seg = -1;           // segment whose stage inputs Y and the sets hold
+    bool recorded = false;
+    hipStream_t stream = nullptr;
+    int nlines() const { return b ? b->nlines : 1; }
+    hipStream_t strm() const { return b ? b->stream : stream; }
+    bool alphamax() const { return b ? b->alphamax() : p->alphamax(); }
+    int64_t seg_len(int64_t s) const { return std::min<int64_t>(m, nsteps - s * m); }
+    ~hrw_adjoint_s()
+    {
+        for (auto &c : ckpt) dfree(c);
+        for (auto &y : Y) dfree(y);
+        dfree(lamA); dfree(lamB); dfree(sets);
+        if (own_lines) {
+            dfree(d_off); dfree(d_n); dfree(d_k); dfree(d_h); dfree(d_xmin); dfree(d_rx);
+        }
+        dfree(d_cline); dfree(d_cstart); dfree(d_lchunk0); dfree(d_lnch);
+        for (auto &q : partial) dfree(q);
+        for (auto &q : amax) dfree(q);
+    }
+    Pending owed_by(const double *buf) const
+    {
+        for (auto &e : owed)
+            if (e.first == buf) return e.second;
+        return Pending{};
+    }
+    void set_owed(const double *buf, const Pending &p)
+    {
+        for (auto &e : owed)
+            if (e.first == buf) {
+                e.second = p;
+                return;
+            }
+        owed.emplace_back(buf, p);
+    }
+};
+
+static int adjoint_alloc(hrw_adjoint_s *a, const std::vector<int> &ncells)
+{
+    a->nseg = a->nsteps > 0 ? (a->nsteps + a->m - 1) / a->m : 0;
+    a->ckpt.assign((size_t)a->nseg + 1, nullptr);
+    for (auto &c : a->ckpt) CALL(dalloc(&c, (size_t)a->flat));
+    const int ny = a->nsteps > 0 ? a->order * a->m - 1 : 0;
+    a->Y.assign((size_t)ny, nullptr);
+    for (auto &y : a->Y) CALL(dalloc(&y, (size_t)a->flat));
+    CALL(dalloc(&a->lamA, (size_t)a->flat));
+    CALL(dalloc(&a->lamB, (size_t)a->flat));
+    a->set_words = a->b ? (size_t)a->b->nlines : (size_t)kNSlot * kSlotStride;
+    CALL(dalloc(&a->sets, a->set_words * (size_t)(a->order * a->m + 2)));
+    // chunk table of the reverse pass: output chunks of 62R cells
+    std::vector<int> cl, cs, c0, nc;
+    for (int l = 0; l < (int)ncells.size(); ++l) {
+        c0.push_back((int)cl.size());
+        for (long long st = 0; st < ncells[l]; st += 62LL * a->R) {
+            cl.push_back(l);
+            cs.push_back((int)st);
+        }
+        nc.push_back((int)cl.size() - c0.back());
+    }
+    a->nchunks = (long long)cl.size();
+    CALL(upload(&a->d_cline, cl));
+    CALL(upload(&a->d_cstart, cs));
+    CALL(upload(&a->d_lchunk0, c0));
+    CALL(upload(&a->d_lnch, nc));
+    const size_t L = ncells.size();
+    for (int q = 0; q < hrw_adjoint_s::kSlots; ++q) {
+        CALL(dalloc(&a->partial[q], (size_t)a->nchunks));
+        CALL(dalloc(&a->amax[q], L));
+        fill_u64_kernel<<<grid_for((long long)L, 256, 4096), 256, 0, a->strm()>>>(a->amax[q], (long long)L);
+    }
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(a->strm()));
+    return HRW_OK;
+}
+
+static bool adjoint_problem_ok(const hrw_problem_s *p)
+{
+    return p->kind == PROB_BURGERS1D && p->bc == HRW_BC_PERIODIC &&
+           p->numflux == HRW_FLUX_LAX_FRIEDRICHS && !p->d_width;
+}
+
+extern "C" int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps,
+                                  int ckpt_every, hrw_adjoint_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    if (!adjoint_problem_ok(p))
+        return fail(HRW_ERR_ARG, "the adjoint covers 1D Burgers, periodic, Lax-Friedrichs, linear grid");
+    if (!(order >= 1 && order <= 3))
+        return fail(HRW_ERR_ARG, "Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.");
+    if (!(dt != 0.0) || !std::isfinite(dt)) return fail(HRW_ERR_ARG, "dt must be finite and nonzero");
+    if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
+    if (ckpt_every < 1) return fail(HRW_ERR_ARG, "ckpt_every must be >= 1");
+    if (p->neq >= (1LL << 31) - 64LL * 8) return fail(HRW_ERR_ARG, "the adjoint needs ncells < 2^31 - 512");
+    auto *a = new (std::nothrow) hrw_adjoint_s;
+    if (!a) return fail(HRW_ERR_ALLOC, "out of host memory");
+    std::unique_ptr<hrw_adjoint_s> guard(a);
+    a->p = p;
+    a->order = order;
+    a->dt = dt;
+    a->nsteps = nsteps;
+    a->m = (int)std::max<int64_t>(1, std::min<int64_t>(ckpt_every, nsteps));
+    a->flat = p->neq;
+    a->own_lines = true;
+    CALL(upload(&a->d_off, std::vector<long long>{0}));
+    CALL(upload(&a->d_n, std::vector<int>{(int)p->neq}));
+    CALL(upload(&a->d_k, std::vector<int>{p->k}));
+    CALL(upload(&a->d_h, std::vector<double>{dt}));
+    CALL(upload(&a->d_xmin, std::vector<double>{p->xmin}));
+    CALL(upload(&a->d_rx, std::vector<double>{p->rx}));
+    CALL(adjoint_alloc(a, std::vector<int>{(int)p->neq}));
+    *out = guard.release();
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_every,
+                                        hrw_adjoint_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (b->numflux != HRW_FLUX_LAX_FRIEDRICHS)
+        return fail(HRW_ERR_ARG, "the adjoint covers the Lax-Friedrichs flux");
+    if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
+    if (ckpt_every < 1) return fail(HRW_ERR_ARG, "ckpt_every must be >= 1");
+    auto *a = new (std::nothrow) hrw_adjoint_s;
+    if (!a) return fail(HRW_ERR_ALLOC, "out of host memory");
+    std::unique_ptr<hrw_adjoint_s> guard(a);
+    a->b = b;
+    a->order = 3;
+    a->nsteps = nsteps;
+    a->m = (int)std::max<int64_t>(1, std::min<int64_t>(ckpt_every, nsteps));
+    a->flat = b->total;
+    a->d_off = b->d_off;
+    a->d_n = b->d_n;
+    a->d_k = b->d_k;
+    a->d_h = b->d_h;
+    a->d_xmin = b->d_xmin;
+    a->d_rx = b->d_rx;
+    CALL(adjoint_alloc(a, b->h_n));
+    *out = guard.release();
+    return HRW_OK;
+}
+
+extern "C" int hrw_adjoint_destroy(hrw_adjoint_t a)
+{
+    delete a;
+    return HRW_OK;
+}
+
+extern "C" int hrw_adjoint_set_stream(hrw_adjoint_t a, hrw_stream_t stream)
+{
+    if (!a) return fail(HRW_ERR_ARG, "null adjoint");
+    if (a->b) return fail(HRW_ERR_ARG, "a batch adjoint runs on the batch's stream (hrw_batch_set_stream)");
+    a->stream = S(stream);
+    return HRW_OK;
+}
+
+extern "C" int hrw_adjoint_info(hrw_adjoint_t a, int64_t *neq, int64_t *nsteps, int *ckpt_every,
+                                int64_t *device_bytes)
+{
+    if (!a) return fail(HRW_ERR_ARG, "null adjoint");
+    if (neq) *neq = a->flat;
+    if (nsteps) *nsteps = a->nsteps;
+    if (ckpt_every) *ckpt_every = a->m;
+    if (device_bytes)
+        *device_bytes = (int64_t)((a->ckpt.size() + a->Y.size() + 2) * (size_t)a->flat * sizeof(double) +
+                                  a->set_words * (size_t)(a->order * a->m + 2) * 8);
+    return HRW_OK;
+}
+
+static int adjoint_seed(hrw_adjoint_s *a, const double *u)
+{
+    if (!a->alphamax()) return HRW_OK;
+    hipStream_t s = a->strm();
+    if (a->b) {
+        hrw_batch_s *b = a->b;
+        fill_u64_kernel<<<grid_for(2LL * b->nlines, 256, 4096), 256, 0, s>>>(a->sets, 2LL * b->nlines);
+        batch_maxabs_kernel<<<(unsigned)((b->nchunks * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+            u, b->d_off, b->d_n, b->d_cline, b->d_cstart, b->nchunks, b->chunk_R, a->sets);
+    } else {
+        clear_slots_kernel<<<grid_for(2 * kNSlot, 256), 256, 0, s>>>(a->sets, 2);
+        maxabs_kernel<<<grid_for(a->p->neq, kBlock * 8, 2048), kBlock, 0, s>>>(u, a->p->neq, a->sets);
+    }
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+static double *seg_state(hrw_adjoint_s *a, int64_t s, int64_t q)
+{
+    const int64_t last = a->order * a->seg_len(s);
+    if (q == 0) return a->ckpt[(size_t)s];
+    if (q == last) return a->ckpt[(size_t)s + 1];
+    return a->Y[(size_t)q - 1];
+}
+
+// Forward over segment s from its checkpoint; `full` = also the last launch (whose output is
+// the next checkpoint).
+static int adjoint_segment(hrw_adjoint_s *a, int64_t s, bool full)
+{
+    static const int stage_id[3][3] = {{RK1, 0, 0}, {RK2_1, RK2_2, 0}, {RK3_1, RK3_2, RK3_3}};
+    const int64_t nq = a->order * a->seg_len(s) - (full ? 0 : 1);
+    CALL(adjoint_seed(a, seg_state(a, s, 0)));
+    const size_t W = a->set_words;
+    for (int64_t q = 0; q < nq; ++q) {
+        const int st = (int)(q % a->order);
+        const double *us = seg_state(a, s, q), *u0 = seg_state(a, s, q - st);
+        double *uo = seg_state(a, s, q + 1);
+        unsigned long long *ain = a->sets + (size_t)q * W, *aout = ain + W, *aclr = aout + W;
+        const int stage = stage_id[a->order - 1][st];
+        if (a->b) {
+            CALL(batch_stage_sets(a->b, ain, aout, aclr, stage, us, u0, uo));
+        } else {
+            StageIO io;
+            io.us = us;
+            io.u0 = u0;
+            io.uo = uo;
+            CALL(run_stage_sets(a->p, ain, aout, aclr, stage, a->dt, io, a->stream));
+        }
+    }
+    return HRW_OK;
+}
+
+extern "C" int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN)
+{
+    if (!a) return fail(HRW_ERR_ARG, "null adjoint");
+    if (!u0 || !uN) return fail(HRW_ERR_ARG, "null state");
+    const size_t bytes = (size_t)a->flat * sizeof(double);
+    hipStream_t s = a->strm();
+    HIP_TRY(hipMemcpyAsync(a->ckpt[0], u0, bytes, hipMemcpyDeviceToDevice, s));
+    a->recorded = false;
+    for (int64_t g = 0; g < a->nseg; ++g) CALL(adjoint_segment(a, g, true));
+    a->y_seg = a->nseg - 1;
+    HIP_TRY(hipMemcpyAsync(uN, a->ckpt[(size_t)a->nseg], bytes, hipMemcpyDeviceToDevice, s));
+    a->recorded = true;
+    return HRW_OK;
+}
+
+template <int R, bool AMAX>
+static void ladj(const AdjArgs &g, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((g.nchunks * 64 + kBlock - 1) / kBlock);
+    adjoint_stage_kernel<R, AMAX><<<grid, kBlock, 0, s>>>(g);
+}
+
+// out = ca*lam + cc*h*J(X[q])^T lam + e1*x1 + e2*x2 over every line.
+static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *lam, double *out,
+                         double ca, double cc, const double *x1, double e1, const double *x2,
+                         double e2)
+{
+    AdjArgs g{};
+    g.s = seg_state(a, s, q);
+    g.lam = lam;
+    g.x1 = x1;
+    g.x2 = x2;
+    g.out = out;
+    g.off = a->d_off;
+    g.n = a->d_n;
+    g.k = a->d_k;
+    g.h = a->d_h;
+    g.xmin = a->d_xmin;
+    g.rx = a->d_rx;
+    g.chunk_line = a->d_cline;
+    g.chunk_start = a->d_cstart;
+    g.line_chunk0 = a->d_lchunk0;
+    g.line_nchunks = a->d_lnch;
+    g.nchunks = a->nchunks;
+    g.a = ca;
+    g.c = cc;
+    g.e1 = e1;
+    g.e2 = e2;
+    g.eps = a->b ? a->b->eps : a->p->eps;
+    g.alpha = a->b ? a->b->alpha : a->p->alpha;
+    g.aset = a->sets + (size_t)q * a->set_words;
+    if (a->b) {
+        g.aline_stride = 1;
+        g.anpart = 1;
+        g.apart_stride = 0;
+    } else {
+        g.aline_stride = 0;
+        g.anpart = kNSlot;
+        g.apart_stride = kSlotStride;
+    }
+    const bool amax = a->alphamax();
+    if (amax) {
+        if (a->tag >= 0xfffffff0u) {   // tags wrap: start over from clean slots
+            for (auto *q : a->amax)
+                fill_u64_kernel<<<grid_for(a->nlines(), 256, 4096), 256, 0, a->strm()>>>(q, a->nlines());
+            a->tag = 0;
+        }
+        const int slot = (int)(a->launches % hrw_adjoint_s::kSlots);
+        g.partial_out = a->partial[slot];
+        g.amax_out = a->amax[slot];
+        g.tag = ++a->tag;
+        g.pend[0] = a->owed_by(lam);
+        g.pend[1] = x1 ? a->owed_by(x1) : Pending{};
+        g.pend[2] = x2 ? a->owed_by(x2) : Pending{};
+        Pending mine{};
+        mine.partial = g.partial_out;
+        mine.amax = g.amax_out;
+        mine.s = g.s;
+        mine.cf = cc;
+        mine.tag = g.tag;
+        a->set_owed(out, mine);   // out's earlier debt (if out aliases x1/x2) is settled here
+        ++a->launches;
+    }
+    if (a->R == 8) {
+        if (amax) ladj<8, true>(g, a->strm());
+        else ladj<8, false>(g, a->strm());
+    } else {
+        if (amax) ladj<4, true>(g, a->strm());
+        else ladj<4, false>(g, a->strm());
+    }
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
+extern "C" int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g0)
+{
+    if (!a) return fail(HRW_ERR_ARG, "null adjoint");
+    if (!gN || !g0) return fail(HRW_ERR_ARG, "null cotangent");
+    if (!a->recorded) return fail(HRW_ERR_STATE, "hrw_adjoint_backward needs a recorded hrw_adjoint_forward");
+    if (((uintptr_t)g0 & 15u) != 0) return fail(HRW_ERR_ARG, "the cotangent must be 16-byte aligned");
+    const size_t bytes = (size_t)a->flat * sizeof(double);
+    hipStream_t st = a->strm();
+    double *lam = g0, *A = a->lamA, *B = a->lamB;
+    a->owed.clear();
+    if (gN != g0) HIP_TRY(hipMemcpyAsync(g0, gN, bytes, hipMemcpyDeviceToDevice, st));
+    for (int64_t s = a->nseg - 1; s >= 0; --s) {
+        if (a->y_seg != s) {
+            CALL(adjoint_segment(a, s, false));
+            a->y_seg = s;
+        }
+        for (int64_t j = a->seg_len(s) - 1; j >= 0; --j) {
+            const int64_t q = a->order * j;
+            switch (a->order) {
+            case 3:   // u' = (u + 2 u2 + 2h L(u2))/3, u2 = (3u + u1 + h L(u1))/4, u1 = u + h L(u)
+                CALL(adjoint_stage(a, s, q + 2, lam, A, 2.0 / 3.0, 2.0 / 3.0, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q + 1, A, B, 0.25, 0.25, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q, B, lam, 1.0, 1.0, A, 0.75, lam, 1.0 / 3.0));
+                break;
+            case 2:   // u' = (u + u1 + h L(u1))/2, u1 = u + h L(u)
+                CALL(adjoint_stage(a, s, q + 1, lam, A, 0.5, 0.5, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q, A, lam, 1.0, 1.0, lam, 0.5, nullptr, 0));
+                break;
+            default:  // u' = u + h L(u)
+                CALL(adjoint_stage(a, s, q, lam, A, 1.0, 1.0, nullptr, 0, nullptr, 0));
+                std::swap(lam, A);
+                break;
+            }
+        }
+    }
+    // the correction owed by the last launch's output
+    const Pending last = a->owed_by(lam);
+    if (last.partial) {
+        adjoint_fixup_kernel<<<a->nlines(), 64, 0, st>>>(last, lam, a->d_off, a->d_h, a->d_lchunk0,
+                                                        a->d_lnch, a->nlines());
+        LAUNCH_CHECK();
+    }
+    a->owed.clear();
+    // order 1 ping-pongs between g0 and the scratch buffer: the result may sit in either
+    if (lam != g0) HIP_TRY(hipMemcpyAsync(g0, lam, bytes, hipMemcpyDeviceToDevice, st));
+    a->lamA = lam == g0 ? A : lam;
+    return HRW_OK;
+}
+
+// ---- recording rktvd%integrate for the adjoint (fused periodic Burgers problems) ----------
+static int ode_free_adjoint(hrw_ode_s *o)
+{
+    delete o->adj;
+    o->adj = nullptr;
+    return HRW_OK;
+}
+
+// The loop of rk_integrate with its step count known up front (the same t accumulation and
+// is_done test), run through a recording adjoint object.
+static int rk_integrate_taped(hrw_ode_s *o, double *u, double *t, double tout, double dt,
+                              int itask)
+{
+    int64_t nsteps = 0;
+    double tt = *t;
+    for (;;) {                                         // :137-171
+        tt = tt + dt;
+        ++nsteps;
+        if (is_done(tt, tout, dt) || itask == 2) break;
+    }
+    hrw_adjoint_s *a = o->adj;
+    if (!a || a->nsteps != nsteps || a->dt != dt || a->order != o->order ||
+        a->m != (int)std::max<int64_t>(1, std::min<int64_t>(o->tape_every, nsteps))) {
+        ode_free_adjoint(o);
+        CALL(hrw_adjoint_create(o->p, o->order, dt, nsteps, o->tape_every, &o->adj));
+        a = o->adj;
+    }
+    a->stream = o->stream;
+    CALL(hrw_adjoint_forward(a, u, u));
+    *t = tt;
+    o->fevals += o->order * (int)nsteps;
+    if (o->istate == 1) o->istate = 2;                 // :176
+    return HRW_OK;
+}
+
+extern "C" int hrw_ode_tape(hrw_ode_t ode, int ckpt_every)
+{
+    if (!ode) return fail(HRW_ERR_ARG, "null ode");
+    if (ckpt_every < 0) return fail(HRW_ERR_ARG, "ckpt_every must be >= 0");
+    if (ckpt_every > 0) {
+        if (ode->method != METHOD_RK || !ode->p)
+            return fail(HRW_ERR_ARG, "recording needs rktvd on a device problem");
+        if (!adjoint_problem_ok(ode->p))
+            return fail(HRW_ERR_ARG, "the adjoint covers 1D Burgers, periodic, Lax-Friedrichs, linear grid");
+    } else {
+        ode_free_adjoint(ode);
+    }
+    ode->tape_every = ckpt_every;
+    return HRW_OK;
+}
+
+extern "C" int hrw_ode_adjoint(hrw_ode_t ode, double *lambda)
+{
+    if (!ode) return fail(HRW_ERR_ARG, "null ode");
+    if (!lambda) return fail(HRW_ERR_ARG, "null lambda");
+    if (!ode->adj || !ode->adj->recorded)
+        return fail(HRW_ERR_STATE, "adjoint: no recorded integrate call (enable it with tape)");
+    ode->adj->stream = ode->stream;
+    return hrw_adjoint_backward(ode->adj, lambda, lambda);
 }

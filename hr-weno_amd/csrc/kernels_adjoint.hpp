@@ -1,0 +1,443 @@
+// kernels_adjoint.hpp -- reverse (vector-Jacobian) pass of one TVD stage for periodic 1D
+// Burgers lines with the Lax-Friedrichs flux: the discrete adjoint of the fused stage pass.
+//
+// Forward stage (src/hrweno_tvdode.f90:136-171):  u_out = A*us + B*u_n + C*h*L(us), with
+// L(us)_i = -(F_{i+1/2} - F_{i-1/2})/w_i, F = LF(vr_i, vl_{i+1}) (src/hrweno_fluxes.f90:43),
+// vr/vl the WENO reconstruction of us (src/hrweno_weno.f90:174-216).  The reverse pass
+// computes, for a cotangent lam of u_out,
+//     out = a*lam + c*h*J(us)^T lam + e1*x1 + e2*x2
+// (a = A, c = C; x1, x2 are pointwise cotangents folded in by the driver, e.g. the B terms of
+// the later stages of the step).  Per cell i, with phi = lam/w:
+//     F-bar_{i+1/2} = phi_{i+1} - phi_i
+//     vr-bar_i = F-bar_{i+1/2} (vr_i + alpha)/2,   vl-bar_i = F-bar_{i-1/2} (vl_i - alpha)/2
+//     alpha-bar = sum_i (F-bar_{i+1/2} vr_i - F-bar_{i-1/2} vl_i)/2
+// then through the nonlinear weights: d vr / d p_r = -2 w_r (q_r - vr) / p_r with
+// p_r = 12 (eps + beta_r) and q_r the candidate values, and through beta_r and q_r (linear /
+// quadratic in the window) to the window cells.  With alpha = max|us| (a subgradient at the
+// first cell attaining the maximum) alpha-bar lands on that cell with the sign of us there.
+//
+// Layout.  Wavefront c of the grid takes chunk c of a chunk table: OUTPUT cells
+// [W, W + 62R) of line `line` (W = start[c]).  Lane l (0..63) owns the R consecutive cells
+// from c0 = W + (l-1)*R; lanes 0 and 63 are halo lanes whose results are only used through
+// the neighbour exchange.  Each lane scatters the window cotangents of its R reconstructions
+// into R + 2(K-1) accumulators and hands the K-1 cells that spill past each end to the
+// neighbouring lane (ds_bpermute).  Anything the wave-edge lanes get wrong from a missing
+// neighbour stays within their own cells when R >= 2K-2, which R >= 4 guarantees.
+// Positions outside [0, n) stand for their periodic images (every (cell, offset) pair of the
+// periodic stencil is counted once, also for lines shorter than the stencil).
+//
+// alpha-bar is a reduction over a whole line, so its correction to the output (at the argmax
+// cell) is known only when the launch has finished.  Each wave writes its partial sum to
+// partial_out[c] and, if it holds the argmax, atomicMax-es the key (tag << 32 | ~cell) into
+// amax_out[line] (tags grow with every launch, so no slot is ever reset and the smallest cell
+// wins).  The launch that next reads this output (as lam, x1 or x2) gets it as a Pending and
+// adds the correction  cf*h * sum(partials) * sign(s_prev[cell])  to the value it loads at
+// that cell, in registers; every wave that needs it sums the partials in the same order.
+// No wave ever waits for another, and no memory fence is needed inside a launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels_1d_fast.hpp"
+
+namespace hrw {
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned wave_umin(unsigned v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+// The not yet applied alpha correction of an earlier launch's output.
+struct Pending {
+    const double *partial;              // that launch's per-chunk partial sums (null: none)
+    const unsigned long long *amax;     // its per-line argmax keys
+    const double *s;                    // its stage input (for the sign at the argmax)
+    double cf;                          // its c (the correction is cf * h[line] * sum)
+    unsigned tag;
+};
+
+struct AdjArgs {
+    const double *s;            // stage input state (flat, line l at off[l])
+    const double *lam;          // cotangent of the stage output
+    const double *x1, *x2;      // pointwise cotangents folded into out (or null)
+    double *out;                // may alias x1 or x2 (read and written by the same lane)
+    const long long *off;
+    const int *n, *k;
+    const double *h, *xmin, *rx;
+    const int *chunk_line, *chunk_start;
+    const int *line_chunk0, *line_nchunks;
+    long long nchunks;
+    double a, c, e1, e2, eps, alpha;
+    const unsigned long long *aset;   // alpha partials: line l, partial p at
+    int aline_stride, anpart, apart_stride;   //   aset[l*aline_stride + p*apart_stride]
+    double *partial_out;        // this launch: one partial alpha-bar per chunk
+    unsigned long long *amax_out;     // this launch: argmax key per line
+    unsigned tag;
+    Pending pend[3];            // corrections owed by lam, x1, x2
+};
+
+// Resolve a pending correction for `line`: the cell j and the amount delta.  Wave-uniform.
+// `wanted` (per lane) says whether this lane holds cell j; the partial sum is only formed
+// when some lane does.
+__device__ __forceinline__ int pending_cell(const Pending &p, int line)
+{
+    if (!p.partial) return -1;
+    const unsigned long long key = p.amax[line];
+    if ((unsigned)(key >> 32) != p.tag) return -1;
+    return (int)(0xffffffffu - (unsigned)(key & 0xffffffffull));
+}
+__device__ __forceinline__ double pending_delta(const Pending &p, int j, long long off, double h,
+                                                int first, int nc)
+{
+    const int lane = threadIdx.x & 63;
+    double sum = 0.0;
+    for (int c = lane; c < nc; c += 64) sum += p.partial[first + c];
+    sum = __shfl(wave_sum(sum), 0, 64);
+    const double sj = p.s[off + j];
+    return p.cf * h * sum * (double)((sj > 0.0) - (sj < 0.0));
+}
+
+// Add a pending correction to the values vals[i] a lane loaded from cells cell[i] (only where
+// `active`).  Must be called by the whole wave.
+template <int R>
+__device__ __forceinline__ void apply_pending(const Pending &p, int line, long long off, double h,
+                                              int first, int nc, const long long *cell,
+                                              bool active, double *vals)
+{
+    const int j = pending_cell(p, line);
+    if (j < 0) return;
+    bool hit = false;
+#pragma unroll
+    for (int i = 0; i < R; ++i) hit = hit || (active && cell[i] == j);
+    if (!__any(hit)) return;
+    const double d = pending_delta(p, j, off, h, first, nc);
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (active && cell[i] == j) vals[i] += d;
+}
+
+
+// Reverse of one K = 3 reconstruction of cell i from its window x[0..4] = v_{i-2..i+2}, for
+// the flux cotangents phi3 = (phi_{i-1}, phi_i, phi_{i+1}).  The triple pieces are rebuilt
+// from the window for each cell (cheaper in registers than carrying them between cells).
+// The window cotangent is added to acc[0..4]; returns the cell's share of alpha-bar.
+__device__ __forceinline__ double recon3_vjp(double eps12, const double *x, const double *phi3,
+                                             double alpha, double *acc)
+{
+    // T(i-2) = x0..x2 (beta_2, right cand P2, left cand P1), T(i-1) = x1..x3 (beta_1, P1, P0),
+    // T(i) = x2..x4 (beta_0, P0, Pm); with D = a - 2b + c, E = a - c:
+    //   12 beta_0 = 13 D^2 + 3 (2D + E)^2, 12 beta_1 = 13 D^2 + 3 E^2, 12 beta_2 = 13 D^2 + 3 (2D - E)^2
+    const double D2 = fma_(-2.0, x[1], x[0]) + x[2], E2 = x[0] - x[2];
+    const double D1 = fma_(-2.0, x[2], x[1]) + x[3], E1 = x[1] - x[3];
+    const double D0 = fma_(-2.0, x[3], x[2]) + x[4], E0 = x[2] - x[4];
+    const double Y2 = fma_(2.0, D2, -E2), X0 = fma_(2.0, D0, E0);
+    const double p2 = eps12 + fma_(3.0 * Y2, Y2, (13.0 * D2) * D2);
+    const double p1 = eps12 + fma_(3.0 * E1, E1, (13.0 * D1) * D1);
+    const double p0 = eps12 + fma_(3.0 * X0, X0, (13.0 * D0) * D0);
+    // 6x candidates: right P0(T(i)), P1(T(i-1)), P2(T(i-2)); left Pm(T(i)), P0(T(i-1)), P1(T(i-2))
+    const double cr0 = fma_(5.0, x[3], fma_(2.0, x[2], -x[4]));
+    const double cr1 = fma_(5.0, x[2], fma_(2.0, x[3], -x[1]));
+    const double cr2 = fma_(11.0, x[2], fma_(-7.0, x[1], x[0] + x[0]));
+    const double cl0 = fma_(11.0, x[2], fma_(-7.0, x[3], x[4] + x[4]));
+    const double cl1 = fma_(5.0, x[2], fma_(2.0, x[1], -x[3]));
+    const double cl2 = fma_(5.0, x[1], fma_(2.0, x[2], -x[0]));
+    const double Q0 = p0 * p0, Q1 = p1 * p1, Q2 = p2 * p2;
+    const double m0 = Q1 * Q2, m1 = Q0 * Q2, m2 = Q0 * Q1;
+    const double a0 = 3.0 * m0, a1 = 6.0 * m1, l2 = 3.0 * m2;
+    const double sr = 6.0 * (a0 + a1 + m2), sl = 6.0 * (m0 + a1 + l2);
+    const double ir = rcp_nr(sr), il = rcp_nr(sl);
+    const double vr = fma_(a0, cr0, fma_(a1, cr1, m2 * cr2)) * ir;
+    const double vl = fma_(m0, cl0, fma_(a1, cl1, l2 * cl2)) * il;
+    const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
+    const double sgr = (0.5 * Gp * (vr + alpha)) * ir, sgl = (0.5 * Gm * (vl - alpha)) * il;
+    const double vr6 = 6.0 * vr, vl6 = 6.0 * vl;
+    // cotangent of p_r = 12 (eps + beta_r):  -2/p_r [ w-numerator_r sgr (cr_r - 6vr) + ... ]
+    const double ipi = -2.0 * rcp_nr(p0 * p1 * p2);
+    const double pb0 = (p1 * p2 * ipi) * fma_(a0 * sgr, cr0 - vr6, (m0 * sgl) * (cl0 - vl6));
+    const double pb1 = (p0 * p2 * ipi) * fma_(a1 * sgr, cr1 - vr6, (a1 * sgl) * (cl1 - vl6));
+    const double pb2 = (p0 * p1 * ipi) * fma_(m2 * sgr, cr2 - vr6, (l2 * sgl) * (cl2 - vl6));
+    // T(i-2): beta_2 (D, Y = 2D - E), candidates P2 (right, m2) and P1 (left, l2)
+    {
+        const double Db = 26.0 * D2 * pb2, Yb = 6.0 * Y2 * pb2;
+        const double Pr = m2 * sgr, Pl = l2 * sgl;
+        acc[0] += fma_(2.0, Pr, Db + Yb - Pl);
+        acc[1] += fma_(-7.0, Pr, fma_(5.0, Pl, fma_(-4.0, Yb, -2.0 * Db)));
+        acc[2] += fma_(11.0, Pr, fma_(2.0, Pl, fma_(3.0, Yb, Db)));
+    }
+    // T(i-1): beta_1 (D, E), candidates P1 (right, a1) and P0 (left, a1)
+    {
+        const double Db = 26.0 * D1 * pb1, Eb = 6.0 * E1 * pb1;
+        const double Pr = a1 * sgr, Pl = a1 * sgl;
+        acc[1] += fma_(2.0, Pl, Db + Eb - Pr);
+        acc[2] += fma_(5.0, Pr + Pl, -2.0 * Db);
+        acc[3] += fma_(2.0, Pr, Db - Eb - Pl);
+    }
+    // T(i): beta_0 (D, X = 2D + E), candidates P0 (right, a0) and Pm (left, m0)
+    {
+        const double Db = 26.0 * D0 * pb0, Xb = 6.0 * X0 * pb0;
+        const double Pr = a0 * sgr, Pl = m0 * sgl;
+        acc[2] += fma_(11.0, Pl, fma_(2.0, Pr, fma_(3.0, Xb, Db)));
+        acc[3] += fma_(-7.0, Pl, fma_(5.0, Pr, fma_(-4.0, Xb, -2.0 * Db)));
+        acc[4] += fma_(2.0, Pl, Db + Xb - Pr);
+    }
+    return 0.5 * fma_(Gp, vr, -(Gm * vl));
+}
+
+// K = 2 from the pairs (v_{i-1}, v_i) and (v_i, v_{i+1}); acc[0..2] = cells i-1..i+1.
+__device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pair2 &p0,
+                                           double &vr, double &vl, double *phi3, double alpha,
+                                           double *acc, double &abar)
+{
+    const double q0 = eps + p0.B, q1 = eps + p1.B;
+    const double Q0 = q0 * q0, Q1 = q1 * q1;
+    const double a0 = 2.0 * Q1, b1 = 2.0 * Q0;
+    const double sr = 2.0 * (a0 + Q0), sl = 2.0 * (Q1 + b1);
+    const double nr = fma_(a0, p0.S, Q0 * p1.Rr);
+    const double nl = fma_(Q1, p0.Lr, b1 * p1.S);
+    const double ir = rcp_nr(sr), il = rcp_nr(sl);
+    vr = nr * ir;
+    vl = nl * il;
+    const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
+    const double gr = 0.5 * Gp * (vr + alpha), gl = 0.5 * Gm * (vl - alpha);
+    abar = 0.5 * fma_(Gp, vr, -(Gm * vl));
+    const double sgr = gr * ir, sgl = gl * il;
+    const double vr2 = 2.0 * vr, vl2 = 2.0 * vl;
+    const double iq = rcp_nr(q0 * q1);
+    // q-bar_r = -2/q_r [ (2x right weight numerator) sgr (cand - 2vr) + ... ]
+    const double qb0 = -2.0 * q1 * iq * fma_(a0 * sgr, p0.S - vr2, (Q1 * sgl) * (p0.Lr - vl2));
+    const double qb1 = -2.0 * q0 * iq * fma_(Q0 * sgr, p1.Rr - vr2, (b1 * sgl) * (p1.S - vl2));
+    // beta = d^2 with d = b - a = (Rr - S)/2, so d-bar = 2 d q-bar moves -1 to a, +1 to b
+    // pair (v_i, v_{i+1}): S = a+b, Lr = 3a-b
+    {
+        const double Sb = a0 * sgr, Lb = Q1 * sgl;
+        const double dd = (p0.Rr - p0.S) * qb0;
+        acc[1] += fma_(3.0, Lb, Sb - dd);
+        acc[2] += Sb - Lb + dd;
+    }
+    // pair (v_{i-1}, v_i): Rr = 3b-a, S = a+b
+    {
+        const double Rb = Q0 * sgr, Sb = b1 * sgl;
+        const double dd = (p1.Rr - p1.S) * qb1;
+        acc[0] += Sb - Rb - dd;
+        acc[1] += fma_(3.0, Rb, Sb + dd);
+    }
+}
+
+// One wavefront's chunk of a line (see the header comment).  Returns the lane's share of
+// alpha-bar over its owned cells; *cand = the smallest owned cell with |us| == alpha.
+template <int K, int R, bool ALPHAMAX>
+__device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *s,
+                                                const double *lam, const double *x1,
+                                                const double *x2, double *out, int n, int W,
+                                                double h, double xmin, double rx, double alpha,
+                                                int line, long long off, int first, int nc,
+                                                unsigned *cand)
+{
+    constexpr int H = K - 1;
+    constexpr int NV = R + 2 * H;
+    static_assert(R >= 4, "halo lanes need R >= 2K-2");
+    const int lane = threadIdx.x & 63;
+    const long long c0 = (long long)W + (long long)(lane - 1) * R;
+    const bool owned = lane >= 1 && lane <= 62;
+    const bool interior = (long long)W - R - H >= 0 && (long long)W + 63LL * R + H <= n;
+
+    double v[NV], mu[R], phi[R + 2], acc[NV];
+    if (interior) {
+        load_run<R>(s + c0, v + H);
+        load_run<R>(lam + c0, mu);
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            v[i] = __shfl_up(v[R + i], 1, 64);
+            v[H + R + i] = __shfl_down(v[H + i], 1, 64);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = s[cell_index(c0 - H + i, n, true)];
+#pragma unroll
+        for (int i = 0; i < R; ++i) mu[i] = lam[cell_index(c0 + i, n, true)];
+    }
+    long long cell[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) cell[i] = interior ? c0 + i : cell_index(c0 + i, n, true);
+    apply_pending<R>(a.pend[0], line, off, h, first, nc, cell, true, mu);
+    // phi = lam / width for the lane's cells, neighbours' from the adjacent lanes
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const long long g = cell[i];
+        const double w = linear_edge(xmin, rx, g + 1) - linear_edge(xmin, rx, g);   // src/hrweno_grids.f90:76-78,247
+        phi[i + 1] = mu[i] * rcp_nr(w);
+    }
+    phi[0] = __shfl_up(phi[R], 1, 64);
+    phi[R + 1] = __shfl_down(phi[1], 1, 64);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+
+    double abar = 0.0;
+    unsigned cnd = 0xffffffffu;
+    if constexpr (K == 3) {
+        const double eps12 = 12.0 * a.eps;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const double ab = recon3_vjp(eps12, v + t, phi + t, alpha, acc + t);
+            if (owned && c0 + t < n) {
+                abar += ab;
+                if (ALPHAMAX && fabs(v[t + 2]) == alpha && cnd == 0xffffffffu) cnd = (unsigned)(c0 + t);
+            }
+            __builtin_amdgcn_sched_barrier(0);   // one cell at a time: bounds the live values
+        }
+    } else if constexpr (K == 2) {
+        Pair2 p1 = pair2(v[0], v[1]);
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const Pair2 p0 = pair2(v[t + 1], v[t + 2]);
+            double vr, vl, ab;
+            recon2_vjp(a.eps, p1, p0, vr, vl, phi + t, alpha, acc + t, ab);
+            if (owned && c0 + t < n) {
+                abar += ab;
+                if (ALPHAMAX && fabs(v[t + 1]) == alpha && cnd == 0xffffffffu) cnd = (unsigned)(c0 + t);
+            }
+            p1 = p0;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const double vv = v[t];
+            const double Gp = phi[t + 2] - phi[t + 1], Gm = phi[t + 1] - phi[t];
+            acc[t] += 0.5 * Gp * (vv + alpha) + 0.5 * Gm * (vv - alpha);
+            if (owned && c0 + t < n) {
+                abar += 0.5 * (Gp - Gm) * vv;
+                if (ALPHAMAX && fabs(vv) == alpha && cnd == 0xffffffffu) cnd = (unsigned)(c0 + t);
+            }
+        }
+    }
+    // hand the spill-over cells to the neighbours: acc[j] <-> cell c0 - H + j
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double fromR = __shfl_down(acc[i], 1, 64);           // right lane's cells c0+R+i-H...
+        const double fromL = __shfl_up(acc[R + H + i], 1, 64);
+        acc[R + i] += fromR;
+        acc[H + i] += fromL;
+    }
+    // out = a*lam + c*h*acc + e1*x1 + e2*x2 on the owned cells (x1, x2 with their own
+    // pending corrections)
+    double p1v[R], p2v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) p1v[i] = p2v[i] = 0.0;
+    if (owned) {
+        if (interior) {
+            if (x1) load_run<R>(x1 + c0, p1v);
+            if (x2) load_run<R>(x2 + c0, p2v);
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                if (c0 + i < n) {
+                    if (x1) p1v[i] = x1[c0 + i];
+                    if (x2) p2v[i] = x2[c0 + i];
+                }
+            }
+        }
+    }
+    if (x1) apply_pending<R>(a.pend[1], line, off, h, first, nc, cell, owned, p1v);
+    if (x2) apply_pending<R>(a.pend[2], line, off, h, first, nc, cell, owned, p2v);
+    if (owned) {
+        const double ch = a.c * h;
+        double o[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            double t = fma_(ch, acc[H + i], a.a * mu[i]);
+            if (x1) t = fma_(a.e1, p1v[i], t);
+            if (x2) t = fma_(a.e2, p2v[i], t);
+            o[i] = t;
+        }
+        if (interior) {
+            store_run<R>(out + c0, o);
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (c0 + i < n) out[c0 + i] = o[i];
+        }
+    }
+    *cand = cnd;
+    return abar;
+}
+
+// The K = 3 reverse keeps ~220 VGPRs live under the compiler's schedule; two waves per SIMD
+// still overlap one wave's loads with the other's long FP64 stretch.
+#ifndef HRW_ADJ_WAVES_PER_EU
+#define HRW_ADJ_WAVES_PER_EU 2
+#endif
+
+template <int R, bool ALPHAMAX>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_ADJ_WAVES_PER_EU)))
+void adjoint_stage_kernel(AdjArgs a)
+{
+    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= a.nchunks) return;   // whole wavefronts leave together
+    const int line = a.chunk_line[wave];
+    const long long off = a.off[line];
+    const int n = a.n[line];
+    const double h = a.h[line];
+    double alpha = a.alpha;
+    if constexpr (ALPHAMAX) {
+        const unsigned long long *set = a.aset + (long long)line * a.aline_stride;
+        const double p = lane < a.anpart ? __longlong_as_double((long long)set[lane * a.apart_stride]) : 0.0;
+        alpha = wave_max(p);
+    }
+    const double *x1 = a.x1 ? a.x1 + off : nullptr;
+    const double *x2 = a.x2 ? a.x2 + off : nullptr;
+    const int first = a.line_chunk0[line], nc = a.line_nchunks[line];
+    const int W = a.chunk_start[wave];
+    unsigned cand;
+    double ab;
+    switch (a.k[line]) {
+    case 1:
+        ab = chunk_adjoint<1, R, ALPHAMAX>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,
+                                            a.xmin[line], a.rx[line], alpha, line, off, first,
+                                            nc, &cand);
+        break;
+    case 2:
+        ab = chunk_adjoint<2, R, ALPHAMAX>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,
+                                            a.xmin[line], a.rx[line], alpha, line, off, first,
+                                            nc, &cand);
+        break;
+    default:
+        ab = chunk_adjoint<3, R, ALPHAMAX>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,
+                                            a.xmin[line], a.rx[line], alpha, line, off, first,
+                                            nc, &cand);
+        break;
+    }
+    if constexpr (ALPHAMAX) {
+        const double wsum = wave_sum(ab);
+        const unsigned wmin = wave_umin(cand);
+        if (lane == 0) {
+            a.partial_out[wave] = wsum;
+            if (wmin != 0xffffffffu)
+                atomicMax(a.amax_out + line,
+                          ((unsigned long long)a.tag << 32) | (unsigned long long)(0xffffffffu - wmin));
+        }
+    }
+}
+
+// Apply a pending correction to a final output: one wave per line.
+__global__ __launch_bounds__(64) void adjoint_fixup_kernel(Pending p, double *out,
+                                                           const long long *off, const double *h,
+                                                           const int *line_chunk0,
+                                                           const int *line_nchunks, int nlines)
+{
+    const int line = blockIdx.x;
+    if (line >= nlines) return;
+    const int j = pending_cell(p, line);
+    if (j < 0) return;
+    const double d = pending_delta(p, j, off[line], h[line], line_chunk0[line], line_nchunks[line]);
+    if ((threadIdx.x & 63) == 0) out[off[line] + j] += d;
+}
+
+}  // namespace hrw

@@ -210,6 +210,17 @@ module hrweno_capi
          integer(c_int64_t), intent(out) :: neq
          integer(c_int), intent(out) :: order, fevals, istate
       end function
+
+      integer(c_int) function hrw_ode_tape(ode, ckpt_every) bind(C)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ode
+         integer(c_int), value :: ckpt_every
+      end function
+
+      integer(c_int) function hrw_ode_adjoint(ode, lambda) bind(C)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ode, lambda
+      end function
    end interface
 
 contains

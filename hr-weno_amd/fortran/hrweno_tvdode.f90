@@ -65,6 +65,9 @@ module hrweno_tvdode
    !! Runge-Kutta TVD solver.
    contains
       procedure, pass(self) :: integrate => rktvd_integrate
+      procedure, pass(self) :: tape => rktvd_tape
+      procedure, pass(self) :: adjoint => rktvd_adjoint
+      procedure, pass(self) :: adjoint_dev => rktvd_adjoint_dev
    end type rktvd
 
    type, extends(tvdode) :: mstvd
@@ -245,6 +248,40 @@ contains
       real(rk), intent(in) :: dt
       call integrate_host(self, u, t, tout, dt, 1)
    end subroutine mstvd_integrate
+
+   subroutine rktvd_tape(self, ckpt_every)
+   !! Record the following %integrate calls for %adjoint, keeping a device checkpoint every
+   !! ckpt_every steps (0 stops recording).  Needs rktvd(prob, order) on a periodic
+   !! Lax-Friedrichs burgers1d problem.  (Added form: the reference has no adjoint.)
+      class(rktvd), intent(inout) :: self
+      integer, intent(in) :: ckpt_every
+      if (hrw_ode_tape(self%h, int(ckpt_every, c_int)) /= 0) call self%error_msg(hrw_message())
+   end subroutine rktvd_tape
+
+   subroutine rktvd_adjoint_dev(self, lambda)
+   !! %adjoint on a device array lambda (neq doubles).
+      class(rktvd), intent(inout) :: self
+      type(c_ptr), intent(in) :: lambda
+      if (hrw_ode_adjoint(self%h, lambda) /= 0) call self%error_msg(hrw_message())
+   end subroutine rktvd_adjoint_dev
+
+   subroutine rktvd_adjoint(self, lambda)
+   !! Discrete adjoint of the last recorded %integrate call: lambda holds dJ/du at its final
+   !! state on entry and dJ/du at its initial state on return.
+      class(rktvd), intent(inout) :: self
+      real(rk), intent(inout) :: lambda(:)
+      real(rk), allocatable, target :: w(:)
+      integer(c_size_t) :: bytes
+      if (size(lambda) /= self%neq) call self%error_msg("Invalid input 'lambda': size(lambda) /= neq.")
+      w = lambda
+      bytes = int(self%neq, c_size_t)*8_c_size_t
+      if (hrw_memcpy_htod(self%ubuf, c_loc(w), bytes, c_null_ptr) /= 0) &
+         call self%error_msg(hrw_message())
+      call self%adjoint_dev(self%ubuf)
+      if (hrw_memcpy_dtoh(c_loc(w), self%ubuf, bytes, c_null_ptr) /= 0) &
+         call self%error_msg(hrw_message())
+      lambda = w
+   end subroutine rktvd_adjoint
 
    pure subroutine error_msg(self, msg)
    !! Error method (src/hrweno_tvdode.f90:286-297).

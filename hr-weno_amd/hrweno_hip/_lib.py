@@ -77,6 +77,15 @@ SIGNATURES = {
     "hrw_batch_step_per_line": [P, P, L, PD],
     "hrw_batch_tune": [P, P, PI],
     "hrw_batch_set_cells_per_lane": [P, I],
+    "hrw_adjoint_create": [P, I, D, L, I, C.POINTER(P)],
+    "hrw_batch_adjoint_create": [P, L, I, C.POINTER(P)],
+    "hrw_adjoint_destroy": [P],
+    "hrw_adjoint_set_stream": [P, P],
+    "hrw_adjoint_forward": [P, P, P],
+    "hrw_adjoint_backward": [P, P, P],
+    "hrw_adjoint_info": [P, PL, PL, PI, PL],
+    "hrw_ode_tape": [P, I],
+    "hrw_ode_adjoint": [P, P],
 }
 
 _lib = None

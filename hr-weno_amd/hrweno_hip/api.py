@@ -48,6 +48,18 @@ def _pd(a: np.ndarray):
     return a.ctypes.data_as(_lib.PD)
 
 
+def _release(obj, destroy: str) -> None:
+    """Destroy a library object once; silent while the interpreter shuts down."""
+    h = getattr(obj, "_h", None)
+    if not h:
+        return
+    obj._h = None
+    try:
+        getattr(_lib.lib(), destroy)(h)
+    except Exception:
+        pass
+
+
 def version() -> str:
     return _lib.lib().hrw_version().decode()
 
@@ -79,10 +91,7 @@ class Weno:
         self.uniform_grid = xe is None
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            _lib.lib().hrw_weno_destroy(h)
-            self._h = None
+        _release(self, "hrw_weno_destroy")
 
     @property
     def cnu(self) -> np.ndarray:
@@ -215,9 +224,7 @@ class Problem:
     _h = None
 
     def __del__(self):
-        if self._h:
-            _lib.lib().hrw_problem_destroy(self._h)
-            self._h = None
+        _release(self, "hrw_problem_destroy")
 
     @property
     def neq(self) -> int:
@@ -270,9 +277,7 @@ class _Ode:
     _cb = None
 
     def __del__(self):
-        if self._h:
-            _lib.lib().hrw_ode_destroy(self._h)
-            self._h = None
+        _release(self, "hrw_ode_destroy")
 
     def _wrap(self, fu):
         def trampoline(ctx, t, u, udot, stream):
@@ -362,11 +367,25 @@ class BurgersBatch:
         call("hrw_batch_size", h, C.byref(total), offs.ctypes.data_as(_lib.PL))
         self.total, self.offsets = total.value, offs
         self.t = np.zeros(n.size)
+        self._tstream = None
+
+    # The library runs a batch on a non-default stream of its own (it captures hipGraphs).
+    # Each call orders that stream after the caller's current stream and the caller's stream
+    # after the batch's work, so torch ops and events on the caller's stream see the results.
+    def _enter(self):
+        torch = _torch()
+        if self._tstream is None:
+            self._tstream = torch.cuda.Stream()
+            call("hrw_batch_set_stream", self._h, C.c_void_p(self._tstream.cuda_stream))
+        cur = torch.cuda.current_stream()
+        self._tstream.wait_stream(cur)
+        return cur
+
+    def _leave(self, cur):
+        cur.wait_stream(self._tstream)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().hrw_batch_destroy(self._h)
-            self._h = None
+        _release(self, "hrw_batch_destroy")
 
     def pack(self, lines, device):
         """Flat device state from a list of per-line host arrays."""
@@ -382,21 +401,91 @@ class BurgersBatch:
                 for l in range(self.ncells.size)]
 
     def set_stream(self, stream):
-        call("hrw_batch_set_stream", self._h, C.c_void_p(stream))
+        """Run the batch on a given torch.cuda.Stream (not the default stream)."""
+        self._tstream = stream
+        call("hrw_batch_set_stream", self._h, C.c_void_p(stream.cuda_stream))
 
     def tune(self, u) -> int:
         r = C.c_int()
+        cur = self._enter()
         call("hrw_batch_tune", self._h, _dptr(u), C.byref(r))
+        self._leave(cur)
         return r.value
 
     def set_cells_per_lane(self, r: int):
         call("hrw_batch_set_cells_per_lane", self._h, int(r))
 
     def step(self, u, nsteps: int, graph: bool = True):
+        cur = self._enter()
         call("hrw_batch_step", self._h, _dptr(u), int(nsteps), _pd(self.t), int(bool(graph)))
+        self._leave(cur)
 
     def step_per_line(self, u, nsteps: int):
+        cur = self._enter()
         call("hrw_batch_step_per_line", self._h, _dptr(u), int(nsteps), _pd(self.t))
+        self._leave(cur)
+
+
+class Adjoint:
+    """Discrete adjoint of ``nsteps`` fixed rktvd steps of periodic Lax-Friedrichs Burgers:
+    one line (a :class:`Burgers1D` problem, any order, step ``dt``) or every line of a
+    :class:`BurgersBatch` (RK3, each line's own dt).  ``forward`` records u0 -> u_N with a
+    checkpoint every ``ckpt_every`` steps; ``backward`` maps dJ/du_N to dJ/du_0.  The reference
+    has no adjoint (this is the added differentiation form of src/hrweno_tvdode.f90:136-171)."""
+
+    def __init__(self, system, nsteps: int, dt: Optional[float] = None, order: int = 3,
+                 ckpt_every: int = 20):
+        h = C.c_void_p()
+        self._system = system   # keeps the problem / batch alive
+        self.batch = isinstance(system, BurgersBatch)
+        if self.batch:
+            call("hrw_batch_adjoint_create", system._h, int(nsteps), int(ckpt_every), C.byref(h))
+        else:
+            if dt is None:
+                raise ValueError("dt is required for a single-line adjoint")
+            call("hrw_adjoint_create", system._h, int(order), float(dt), int(nsteps),
+                 int(ckpt_every), C.byref(h))
+        self._h = h
+        self.token = 0          # bumps on every recorded forward
+
+    def __del__(self):
+        _release(self, "hrw_adjoint_destroy")
+
+    def _sync_in(self):
+        if self.batch:   # the batch runs on its own stream, ordered against the caller's
+            self._cur = self._system._enter()
+        else:
+            call("hrw_adjoint_set_stream", self._h, _stream())
+
+    def _sync_out(self):
+        if self.batch:
+            self._system._leave(self._cur)
+
+    def info(self) -> dict:
+        neq, ns, b = C.c_int64(), C.c_int64(), C.c_int64()
+        ck = C.c_int()
+        call("hrw_adjoint_info", self._h, C.byref(neq), C.byref(ns), C.byref(ck), C.byref(b))
+        return {"neq": neq.value, "nsteps": ns.value, "ckpt_every": ck.value,
+                "device_bytes": b.value}
+
+    def forward(self, u0, uN=None):
+        """u_N from u0 (device tensors), recording the checkpoints."""
+        if uN is None:
+            uN = _torch().empty_like(u0)
+        self._sync_in()
+        call("hrw_adjoint_forward", self._h, _dptr(u0), _dptr(uN))
+        self._sync_out()
+        self.token += 1
+        return uN
+
+    def backward(self, gN, g0=None):
+        """dJ/du_0 from dJ/du_N for the last recorded forward (g0 may be gN)."""
+        if g0 is None:
+            g0 = _torch().empty_like(gN)
+        self._sync_in()
+        call("hrw_adjoint_backward", self._h, _dptr(gN), _dptr(g0))
+        self._sync_out()
+        return g0
 
 
 def is_done(t: float, tout: float, dt: float) -> bool:

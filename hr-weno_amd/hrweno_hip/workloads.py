@@ -128,6 +128,35 @@ def config3(nlines: int = 4096, seed: int = 3, steps: int = 200):
     return cases
 
 
+def config5(n: int = 1 << 20, seed: int = 5, noise_seed: int = 55, steps: int = 400):
+    """Adjoint form: config-1 style Burgers data on 2**20 periodic cells, 400 RK3 steps.  The
+    loss is 0.5*|u_N - target|^2 with target = the forward run from u0 + N(0, 0.01) per cell;
+    returns (case, noise)."""
+    e, xc, w = linear_centres(-1.0, 1.0, n)
+    u0 = smooth_step_ic(xc, Rng(seed))
+    noise = Rng(noise_seed).normal(n, 0.0, 0.01)
+    return Burgers1DCase(n=n, u0=u0, dt=cfl_dt(u0, 2.0 / n), steps=steps, width=w,
+                         seed=seed), noise
+
+
+def config5_batch(nlines: int = 256, seed: int = 5, steps: int = 400, lo: int = 1024,
+                  hi: int = 16384):
+    """Batched adjoint form: 256 periodic lines, lengths log-uniform in [1024, 16384], WENO5,
+    config-1 style data with own seed and CFL dt; each line's target from its own N(0, 0.01)
+    perturbation.  Returns (cases, noises)."""
+    rng = Rng(seed)
+    lens = np.rint(np.exp(rng.uniform(nlines, np.log(float(lo)), np.log(float(hi))))).astype(np.int64)
+    cases, noises = [], []
+    for l in range(nlines):
+        n = int(lens[l])
+        e, xc, w = linear_centres(-1.0, 1.0, n)
+        u0 = smooth_step_ic(xc, Rng((seed << 32) + l))
+        cases.append(Burgers1DCase(n=n, u0=u0, k=3, dt=cfl_dt(u0, 2.0 / n), steps=steps,
+                                   width=w, seed=(seed << 32) + l))
+        noises.append(Rng((55 << 32) + l).normal(n, 0.0, 0.01))
+    return cases, noises
+
+
 @dataclass
 class PBE2DCase:
     n1: int

@@ -24,6 +24,7 @@ typedef struct hrw_weno_s *hrw_weno_t;
 typedef struct hrw_problem_s *hrw_problem_t;
 typedef struct hrw_ode_s *hrw_ode_t;
 typedef struct hrw_batch_s *hrw_batch_t;
+typedef struct hrw_adjoint_s *hrw_adjoint_t;
 
 enum hrw_status {
     HRW_OK = 0,
@@ -206,6 +207,34 @@ int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps, double *t)
  * the fastest; or set it directly. */
 int hrw_batch_tune(hrw_batch_t b, const double *u, int *cells_per_lane);
 int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane);
+
+/* ---------------------------------------------------------------- discrete adjoint - */
+/* Reverse mode of nsteps fixed rktvd steps (src/hrweno_tvdode.f90:136-171) of periodic 1D
+ * Burgers with the Lax-Friedrichs flux (example/example1_burgers_1d_fv.f90:72-109 on a
+ * periodic line; src/hrweno_fluxes.f90:22-45; src/hrweno_weno.f90:174-216).  The reference
+ * has no adjoint; this is the added differentiation form.  forward records u0 -> u_N with a
+ * checkpoint every ckpt_every steps; backward maps a cotangent dJ/du_N to dJ/du_0,
+ * recomputing one segment of ckpt_every steps at a time.  With alpha = max|u| the gradient
+ * takes the subgradient at the first cell attaining the maximum.  Device memory:
+ * (nsteps/ckpt_every + 1) + (order*ckpt_every - 1) + 2 states. */
+int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps, int ckpt_every,
+                       hrw_adjoint_t *out);
+/* Every line of a batch (RK3, each line with its own dt); runs on the batch's stream and
+ * needs the batch to outlive it.  States are the batch's flat layout. */
+int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_every, hrw_adjoint_t *out);
+int hrw_adjoint_destroy(hrw_adjoint_t a);
+int hrw_adjoint_set_stream(hrw_adjoint_t a, hrw_stream_t stream);
+int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN);
+/* gN may equal g0; needs a preceding hrw_adjoint_forward (may be called again after it). */
+int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g0);
+int hrw_adjoint_info(hrw_adjoint_t a, int64_t *neq, int64_t *nsteps, int *ckpt_every,
+                     int64_t *device_bytes);
+/* ode%tape(ckpt_every) / ode%adjoint(lambda): the same reverse mode on an rktvd object built
+ * on a fused problem (hrw_rktvd_create).  With ckpt_every > 0 every later hrw_ode_integrate
+ * records its steps (0 stops recording); hrw_ode_adjoint maps lambda (device, neq) from
+ * dJ/du at the end of the last recorded integrate call to dJ/du at its start, in place. */
+int hrw_ode_tape(hrw_ode_t ode, int ckpt_every);
+int hrw_ode_adjoint(hrw_ode_t ode, double *lambda);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,23 @@ int hro_pbe_ms(int n1, int n2, int k, double eps, int bc, double g1a, double g1b
 int hro_linear_ode(int n, const double *a, int order, double *u, double *t,
                    double tout, double dt);
 
+/* ---- discrete adjoint of the periodic Burgers path (hrweno_adjoint.c) ---- */
+/* One evaluation of hro_rhs_burgers (uniform tables). */
+int hro_burgers_rhs(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                    const double *width, const double *v, double *vdot);
+/* out = (dL/dv)^T mu for L = hro_rhs_burgers with bc PERIODIC, the Lax-Friedrichs flux
+ * and alpha = alpha_fixed (>= 0) or max|v| (alpha_fixed < 0). */
+int hro_burgers_rhs_vjp(int n, int k, double eps, double alpha_fixed, const double *width,
+                        const double *v, const double *mu, double *out);
+/* nsteps fixed rktvd steps (order 1..3) of that RHS, u <- u_N, then the reverse sweep:
+ * g in: dJ/du_N, out: dJ/du_0. */
+int hro_burgers_rk_adjoint(int n, int k, double eps, double alpha_fixed, const double *width,
+                           int order, int64_t nsteps, double dt, double *u, double *g);
+/* Test hook: 0 = exact adjoint; 1, 2, 3 plant one mistake each (weights held fixed, the sign
+ * of alpha in the left-face flux derivative, alpha held fixed) so tests can show the checks
+ * catch them. */
+void hro_adjoint_plant(int which);
+
 /* OpenMP thread count used by the per-cell loops (results do not depend on it). */
 void hro_set_threads(int nthreads);
 int hro_get_threads(void);

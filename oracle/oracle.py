@@ -70,6 +70,10 @@ class Oracle:
                                    _D]
         lib.hro_burgers_ms.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P, _D, _D]
         lib.hro_linear_ode.argtypes = [_I, _P, _I, _P, _P, _D, _D]
+        lib.hro_burgers_rhs.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P]
+        lib.hro_burgers_rhs_vjp.argtypes = [_I, _I, _D, _D, _P, _P, _P, _P]
+        lib.hro_burgers_rk_adjoint.argtypes = [_I, _I, _D, _D, _P, _I, _L, _D, _P, _P]
+        lib.hro_adjoint_plant.argtypes = [_I]
         lib.hro_set_threads.argtypes = [_I]
         lib.hro_get_threads.restype = _I
         self.lib = lib
@@ -163,6 +167,38 @@ class Oracle:
         if steps < 0:
             raise ValueError("hro_burgers_rk failed")
         return u, t.value, int(steps), fe.value
+
+    def burgers_rhs(self, v, width, *, k=3, eps=1e-6, flux=FLUX_LF, bc=BC_PERIODIC, alpha=-1.0):
+        """One evaluation of the Burgers RHS L(v)."""
+        v, w = _f64(v), _f64(width)
+        out = np.empty(v.size)
+        if self.lib.hro_burgers_rhs(v.size, k, eps, flux, bc, alpha, _ptr(w), _ptr(v), _ptr(out)):
+            raise ValueError("hro_burgers_rhs failed")
+        return out
+
+    # -- discrete adjoint (periodic Burgers, Lax-Friedrichs) ----------------------------
+    def burgers_rhs_vjp(self, v, mu, width, *, k=3, eps=1e-6, alpha=-1.0):
+        """(dL/dv)^T mu for the periodic Lax-Friedrichs Burgers RHS."""
+        v, mu, w = _f64(v), _f64(mu), _f64(width)
+        out = np.empty(v.size)
+        if self.lib.hro_burgers_rhs_vjp(v.size, k, eps, alpha, _ptr(w), _ptr(v), _ptr(mu), _ptr(out)):
+            raise ValueError("hro_burgers_rhs_vjp failed")
+        return out
+
+    def burgers_rk_adjoint(self, u0, gN, width, *, nsteps, dt, k=3, eps=1e-6, alpha=-1.0,
+                           order=3):
+        """nsteps rktvd steps from u0 and the reverse sweep of gN; returns (u_N, dJ/du0)."""
+        u = _f64(u0).copy()
+        g = _f64(gN).copy()
+        w = _f64(width)
+        if self.lib.hro_burgers_rk_adjoint(u.size, k, eps, alpha, _ptr(w), order, int(nsteps), dt,
+                                           _ptr(u), _ptr(g)):
+            raise ValueError("hro_burgers_rk_adjoint failed")
+        return u, g
+
+    def adjoint_plant(self, which: int) -> None:
+        """Test hook: plant mistake `which` (1..3) in the adjoint, 0 restores it."""
+        self.lib.hro_adjoint_plant(int(which))
 
     def pbe_ms(self, u0, edges1, edges2, *, k=3, eps=1e-6, bc=1, g1=(1.0, 0.5), g2=(0.5, 0.0),
                t0=0.0, tout=1.0, dt=1e-3, order=0):

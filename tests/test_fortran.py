@@ -49,3 +49,9 @@ def test_fortran_tvdode():
 def test_fortran_dropin_example1():
     """example 1 as a user RHS on the drop-in API == the fused device problem, bit for bit."""
     run("test_dropin_burgers")
+
+
+@pytest.mark.gpu
+def test_fortran_adjoint():
+    """rktvd%tape / %adjoint on a fused problem: FD agreement, conserved total, checkpoints."""
+    run("test_adjoint")

@@ -159,14 +159,20 @@ def test_mstvd_burgers_exact(oracle, hh, dev):
 
 @pytest.mark.parametrize("r", [4, 8])
 def test_fast_mstvd_and_rhs(oracle, hh, dev, r):
-    """FAST multistep (two pointwise operands per cell) and rhs-only passes of the 1D kernel."""
+    """FAST multistep (two pointwise operands per cell) and rhs-only passes of the 1D kernel.
+
+    The reference's 5-step multistep method amplifies rounding on this WENO5-LF Burgers line
+    at CFL 0.4: in the oracle itself a few-ulp change of u0 grows to 2e-13 after 20 steps and
+    2e-9 after 100.  So the FAST form is compared over 4 starting + 10 multistep steps, where
+    the problem is still well conditioned (the EXACT form is bitwise at any horizon)."""
     from hrweno_hip import workloads as W
     c = W.config1(n=3000, seed=12)
     p = hh.Burgers1D(c.n, 3, 1e-6, arith="fast").tune(0, r)
     ode = hh.MsTvd(p)
     u = _t(c.u0, dev)
-    t = ode.integrate(u, 0.0, 0.05, c.dt)
-    ou, ot, fe = oracle.burgers_ms(c.u0, c.width, tout=0.05, dt=c.dt)
+    tout = W.steps_tout(0.0, c.dt, 14)
+    t = ode.integrate(u, 0.0, tout, c.dt)
+    ou, ot, fe = oracle.burgers_ms(c.u0, c.width, tout=tout, dt=c.dt)
     assert t == ot and ode.fevals == fe
     assert_close(_h(u), ou)
     L = _h(p.rhs(0.0, _t(c.u0, dev)))
