@@ -228,7 +228,7 @@ def run_adjoint(args, torch, hh, W, dev):
     info = adj.info()
     log(f"adjoint: forward(recording) {ms_f:.2f} ms, backward {ms_b:.2f} ms, plain forward "
         f"{ms_p:.2f} ms, reverse-only {ms_r:.2f} ms")
-    rl = roofline(88.0 / 3 * n, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<4,alphamax>",
+    rl = roofline(88.0 / 3 * n, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<2,alphamax>",
                   "adjoint_2p20")
     config = {"workload": "burgers1d_weno5_lf_rk3_adjoint_2^20cells_400steps", "cells": n,
               "steps": steps, "ckpt_every": info["ckpt_every"], "k": 3, "eps": 1e-6,
@@ -276,7 +276,7 @@ def run_adjoint_batch(args, torch, hh, W, dev):
     full.forward(u0)
     full.backward(gN)
     ms_r = timed(torch, lambda: full.backward(gN))
-    rl = roofline(88.0 / 3 * total, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<4,alphamax>",
+    rl = roofline(88.0 / 3 * total, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<2,alphamax>",
                   "adjoint_batch")
     config = {"workload": "burgers1d_batch_adjoint_256_lines_log_uniform_1024_16384",
               "lines": len(cases), "cells": total, "steps": steps, "k": 3,

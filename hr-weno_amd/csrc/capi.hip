@@ -676,7 +676,7 @@ extern "C" int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, doub
 extern "C" int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane)
 {
     if (!p) return fail(HRW_ERR_ARG, "null problem");
-    if (variant != 0 && variant != 1) return fail(HRW_ERR_ARG, "variant must be 0 or 1");
+    if (variant < 0 || variant > 2) return fail(HRW_ERR_ARG, "variant must be 0, 1 or 2");
     if (cells_per_lane != 4 && cells_per_lane != 8)
         return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
     p->variant = variant;
@@ -739,9 +739,44 @@ static void l1f(const Stage1DArgs &a, hipStream_t s)
     const unsigned grid = (unsigned)((a.n + per_block - 1) / per_block);
     stage1d_fast_kernel<K, FLUX, PER, AMAX, R><<<grid, kBlock, 0, s>>>(a);
 }
+// Persistent form: as many blocks as fit on the device at once (occupancy of this
+// instantiation, asked once), never more than there are chunks.
+static int device_cus()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+template <int K, int FLUX, bool PER, bool AMAX, int R>
+static void l1p(const Stage1DArgs &a, hipStream_t s)
+{
+    static int per_cu = 0;
+    if (!per_cu) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void *>(&stage1d_pf_kernel<K, FLUX, PER, AMAX, R>), kBlock, 0) != hipSuccess ||
+            nb <= 0)
+            nb = 2;
+        per_cu = nb;
+    }
+    const long long chunks = (a.n + 64LL * R - 1) / (64LL * R);
+    const long long want = (chunks + kBlock / 64 - 1) / (kBlock / 64);
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(want, (long long)per_cu * device_cus()));
+    stage1d_pf_kernel<K, FLUX, PER, AMAX, R><<<grid, kBlock, 0, s>>>(a);
+}
 template <int K, int FLUX, bool PER, bool AMAX>
 static void l1f_r(int r, const Stage1DArgs &a, hipStream_t s)
 {
+    if (r < 0) {
+        if (r == -4) l1p<K, FLUX, PER, AMAX, 4>(a, s);
+        else l1p<K, FLUX, PER, AMAX, 8>(a, s);
+        return;
+    }
     if (r == 4) l1f<K, FLUX, PER, AMAX, 4>(a, s);
     else l1f<K, FLUX, PER, AMAX, 8>(a, s);
 }
@@ -770,11 +805,12 @@ static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
 {
     const bool amax0 = p->alphamax();
-    if (p->arith == HRW_ARITH_FAST && p->variant == 0 && a.n < (1LL << 31) - (1LL << 20) &&
+    if (p->arith == HRW_ARITH_FAST && (p->variant == 0 || p->variant == 2) &&
+        a.n < (1LL << 31) - (1LL << 20) &&
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
         const bool per = p->bc == HRW_BC_PERIODIC;
-        const int r = p->cells_per_lane;
+        const int r = p->variant == 2 ? -p->cells_per_lane : p->cells_per_lane;   // < 0: persistent
         if (p->k == 1) l1f_x<1>(p->numflux, per, amax0, r, a, s);
         else if (p->k == 2) l1f_x<2>(p->numflux, per, amax0, r, a, s);
         else l1f_x<3>(p->numflux, per, amax0, r, a, s);
@@ -819,6 +855,7 @@ static void launch_stage2d(const hrw_problem_s *p, const Stage2DArgs &a, hipStre
 struct StageIO {
     const double *us = nullptr, *u0 = nullptr, *uo4 = nullptr, *Lo4 = nullptr;
     double *uo = nullptr, *Lout = nullptr;
+    double *alpha_record = nullptr;   // 1D: receives the Lax-Friedrichs alpha of us
 };
 // The same with the three alpha slot sets given explicitly (read / accumulate / clear).
 static int run_stage_sets(const hrw_problem_s *p, const unsigned long long *ain,
@@ -857,6 +894,7 @@ static int run_stage_sets(const hrw_problem_s *p, const unsigned long long *ain,
             a.alpha_in = ain;
             a.alpha_out = aout;
             a.alpha_clear = aclear;
+            a.alpha_record = io.alpha_record;
         }
         launch_stage1d(p, a, s);
     } else {
@@ -1547,13 +1585,14 @@ extern "C" int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane)
 struct hrw_adjoint_s {
     hrw_problem_s *p = nullptr;   // one line, or
     hrw_batch_s *b = nullptr;     // every line of a batch (which must outlive this object)
-    int order = 3, m = 20, R = 4;
+    int order = 3, m = 20, R = 2;   // R: cells per lane of the reverse kernel (2 or 4)
     int64_t nsteps = 0, nseg = 0, flat = 0;
     double dt = 0.0;
     std::vector<double *> ckpt, Y;
     double *lamA = nullptr, *lamB = nullptr;
     unsigned long long *sets = nullptr;
     size_t set_words = 0;
+    double *alpha_rec = nullptr;  // one line: alpha of X[q], recorded by forward launch q
     // reverse-pass tables (the batch's own line tables are borrowed)
     long long *d_off = nullptr;
     int *d_n = nullptr, *d_k = nullptr;
@@ -1582,7 +1621,7 @@ struct hrw_adjoint_s {
     {
         for (auto &c : ckpt) dfree(c);
         for (auto &y : Y) dfree(y);
-        dfree(lamA); dfree(lamB); dfree(sets);
+        dfree(lamA); dfree(lamB); dfree(sets); dfree(alpha_rec);
         if (own_lines) {
             dfree(d_off); dfree(d_n); dfree(d_k); dfree(d_h); dfree(d_xmin); dfree(d_rx);
         }
@@ -1607,6 +1646,7 @@ struct hrw_adjoint_s {
     }
 };
 
+static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells);
 static int adjoint_alloc(hrw_adjoint_s *a, const std::vector<int> &ncells)
 {
     a->nseg = a->nsteps > 0 ? (a->nsteps + a->m - 1) / a->m : 0;
@@ -1619,7 +1659,18 @@ static int adjoint_alloc(hrw_adjoint_s *a, const std::vector<int> &ncells)
     CALL(dalloc(&a->lamB, (size_t)a->flat));
     a->set_words = a->b ? (size_t)a->b->nlines : (size_t)kNSlot * kSlotStride;
     CALL(dalloc(&a->sets, a->set_words * (size_t)(a->order * a->m + 2)));
-    // chunk table of the reverse pass: output chunks of 62R cells
+    if (!a->b) CALL(dalloc(&a->alpha_rec, (size_t)(a->order * a->m + 1)));
+    CALL(adjoint_chunks(a, ncells));
+    return HRW_OK;
+}
+
+// Chunk table of the reverse pass (output chunks of 62R cells) and the per-chunk slots.
+static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
+{
+    dfree(a->d_cline); dfree(a->d_cstart); dfree(a->d_lchunk0); dfree(a->d_lnch);
+    for (auto &q : a->partial) dfree(q);
+    for (auto &q : a->amax) dfree(q);
+    a->owed.clear();
     std::vector<int> cl, cs, c0, nc;
     for (int l = 0; l < (int)ncells.size(); ++l) {
         c0.push_back((int)cl.size());
@@ -1715,6 +1766,15 @@ extern "C" int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_
     return HRW_OK;
 }
 
+extern "C" int hrw_adjoint_set_cells_per_lane(hrw_adjoint_t a, int cells_per_lane)
+{
+    if (!a) return fail(HRW_ERR_ARG, "null adjoint");
+    if (cells_per_lane != 2 && cells_per_lane != 4)
+        return fail(HRW_ERR_ARG, "cells_per_lane must be 2 or 4");
+    a->R = cells_per_lane;
+    return adjoint_chunks(a, a->b ? a->b->h_n : std::vector<int>{(int)a->p->neq});
+}
+
 extern "C" int hrw_adjoint_destroy(hrw_adjoint_t a)
 {
     delete a;
@@ -1788,6 +1848,7 @@ static int adjoint_segment(hrw_adjoint_s *a, int64_t s, bool full)
             io.us = us;
             io.u0 = u0;
             io.uo = uo;
+            io.alpha_record = a->alpha_rec + q;
             CALL(run_stage_sets(a->p, ain, aout, aclr, stage, a->dt, io, a->stream));
         }
     }
@@ -1844,16 +1905,11 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
     g.e2 = e2;
     g.eps = a->b ? a->b->eps : a->p->eps;
     g.alpha = a->b ? a->b->alpha : a->p->alpha;
-    g.aset = a->sets + (size_t)q * a->set_words;
-    if (a->b) {
-        g.aline_stride = 1;
-        g.anpart = 1;
-        g.apart_stride = 0;
-    } else {
-        g.aline_stride = 0;
-        g.anpart = kNSlot;
-        g.apart_stride = kSlotStride;
-    }
+    g.aline_stride = a->b ? 1 : 0;
+    g.anpart = 1;
+    g.apart_stride = 0;
+    g.aset = a->b ? a->sets + (size_t)q * a->set_words
+                  : reinterpret_cast<const unsigned long long *>(a->alpha_rec + q);
     const bool amax = a->alphamax();
     if (amax) {
         if (a->tag >= 0xfffffff0u) {   // tags wrap: start over from clean slots
@@ -1877,12 +1933,12 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
         a->set_owed(out, mine);   // out's earlier debt (if out aliases x1/x2) is settled here
         ++a->launches;
     }
-    if (a->R == 8) {
-        if (amax) ladj<8, true>(g, a->strm());
-        else ladj<8, false>(g, a->strm());
-    } else {
+    if (a->R == 4) {
         if (amax) ladj<4, true>(g, a->strm());
         else ladj<4, false>(g, a->strm());
+    } else {
+        if (amax) ladj<2, true>(g, a->strm());
+        else ladj<2, false>(g, a->strm());
     }
     LAUNCH_CHECK();
     return HRW_OK;
@@ -1901,7 +1957,9 @@ extern "C" int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g
     if (gN != g0) HIP_TRY(hipMemcpyAsync(g0, gN, bytes, hipMemcpyDeviceToDevice, st));
     for (int64_t s = a->nseg - 1; s >= 0; --s) {
         if (a->y_seg != s) {
-            CALL(adjoint_segment(a, s, false));
+            // the whole segment again (its last launch rewrites ckpt[s+1] with the same
+            // values, and records the alpha of the last stage input)
+            CALL(adjoint_segment(a, s, true));
             a->y_seg = s;
         }
         for (int64_t j = a->seg_len(s) - 1; j >= 0; --j) {

@@ -136,6 +136,7 @@ struct Stage1DArgs {
     const unsigned long long *alpha_in;   // slot set holding max|us| (ALPHAMAX)
     unsigned long long *alpha_out;        // slot set receiving max|output|
     unsigned long long *alpha_clear;      // slot set to clear
+    double *alpha_record;                 // if set: receives the alpha this launch used
 };
 
 // R = kBlock*CPT cells are reconstructed per block; the block owns the TILE = R-2 interior
@@ -159,7 +160,10 @@ __global__ __launch_bounds__(kBlock) void stage1d_kernel(Stage1DArgs a)
     for (int j = tid; j < NLOAD; j += kBlock) s_u[j] = a.us[cell_index(base - K + j, n, PERIODIC)];
     clear_set(a.alpha_clear);
     double alpha = a.alpha;
-    if constexpr (ALPHAMAX) alpha = read_alpha(a.alpha_in, s_tmp);   // contains a barrier
+    if constexpr (ALPHAMAX) {
+        alpha = read_alpha(a.alpha_in, s_tmp);   // contains a barrier
+        if (a.alpha_record && blockIdx.x == 0 && threadIdx.x == 0) *a.alpha_record = alpha;
+    }
     else __syncthreads();
 
     // 2. reconstruct cells base-1 .. base+TILE (local c <-> cell base-1+c)

@@ -168,8 +168,33 @@ __host__ __device__ inline StageCoef stage_coef(int stage, double h)
 // pointwise operands of the stage (0, 1 or 2).  Returns the lane's max|output| (the next
 // stage's alpha contribution).  Requires n < 2^31 and 16-byte aligned arrays (checked on the
 // host).  Must be called by all 64 lanes of the wavefront.
+// What an interior chunk reads from memory, held in registers so that it can be loaded one
+// chunk ahead (the persistent kernel below): the lane's R cells of us and the pointwise
+// operands.  (The K-cell halo that only the two wave-edge lanes load is read when used.)
+template <int K, int R, int NPW>
+struct ChunkData {
+    double run[R];
+    double p1[NPW >= 1 ? R : 1], p2[NPW >= 2 ? R : 1];
+};
+
+__device__ __forceinline__ bool chunk_interior(long long W, int R, int H, long long n)
+{
+    return W >= H && W + 64LL * R + H <= n;
+}
+
+template <int K, int R, int NPW>
+__device__ __forceinline__ void load_chunk(const Stage1DArgs &a, int W, ChunkData<K, R, NPW> &d)
+{
+    const int c0 = W + (int)(threadIdx.x & 63) * R;
+    load_run<R>(a.us + c0, d.run);
+    const double *s1 = a.stage == MS3 ? a.uo4 : a.u0;
+    if constexpr (NPW >= 1) load_run<R>(s1 + c0, d.p1);
+    if constexpr (NPW >= 2) load_run<R>(a.Lo4 + c0, d.p2);
+}
+
 template <int K, int FLUX, bool PERIODIC, int R, int NPW>
-__device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha)
+__device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha,
+                                              const ChunkData<K, R, NPW> *pre = nullptr)
 {
     constexpr int H = K;
     constexpr int NV = R + 2 * H;
@@ -183,11 +208,21 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     double v[NV];
     [[maybe_unused]] double p1[NPW >= 1 ? R : 1], p2[NPW >= 2 ? R : 1];
     const double *s1 = a.stage == MS3 ? a.uo4 : a.u0;
-    const bool interior = W >= H && (long long)W + 64LL * R + H <= n;
+    const bool interior = chunk_interior(W, R, H, n);
     if (interior) {
-        load_run<R>(a.us + c0, v + H);
-        if constexpr (NPW >= 1) load_run<R>(s1 + c0, p1);
-        if constexpr (NPW >= 2) load_run<R>(a.Lo4 + c0, p2);
+        ChunkData<K, R, NPW> d;
+        if (pre) d = *pre;
+        else load_chunk<K, R, NPW>(a, W, d);
+#pragma unroll
+        for (int i = 0; i < R; ++i) v[H + i] = d.run[i];
+        if constexpr (NPW >= 1) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) p1[i] = d.p1[i];
+        }
+        if constexpr (NPW >= 2) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) p2[i] = d.p2[i];
+        }
 #pragma unroll
         for (int i = 0; i < H; ++i) {
             v[i] = __shfl_up(v[R + i], 1, 64);            // lane-1's last H cells
@@ -309,6 +344,56 @@ __device__ __forceinline__ double chunk_stage_any(const Stage1DArgs &a, int W, d
 #define HRW_FAST_WAVES_PER_EU 4
 #endif
 
+// Persistent form: wave w of the grid takes chunks w, w + N, w + 2N, ... (N = waves in the
+// grid) and loads the next interior chunk into a second register set before it computes the
+// current one, so every wave keeps a chunk of loads in flight behind its FP64 work.
+template <int K, int FLUX, bool PERIODIC, int R, int NPW>
+__device__ __forceinline__ double persistent_stage(const Stage1DArgs &a, double alpha)
+{
+    const long long nw = (long long)gridDim.x * (kBlock / 64);
+    const long long nch = (a.n + 64LL * R - 1) / (64LL * R);
+    long long c = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    ChunkData<K, R, NPW> cur, nxt;
+    double m = 0.0;
+    if (c < nch && chunk_interior(c * 64 * R, R, K, a.n)) load_chunk<K, R, NPW>(a, (int)(c * 64 * R), cur);
+    for (; c < nch; c += nw) {
+        const long long c1 = c + nw;
+        if (c1 < nch && chunk_interior(c1 * 64 * R, R, K, a.n)) load_chunk<K, R, NPW>(a, (int)(c1 * 64 * R), nxt);
+        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW>(a, (int)(c * 64 * R), alpha, &cur));
+        cur = nxt;
+    }
+    return m;
+}
+
+template <int K, int FLUX, bool PERIODIC, int R>
+__device__ __forceinline__ double persistent_stage_any(const Stage1DArgs &a, double alpha)
+{
+    if (a.stage == MS3) return persistent_stage<K, FLUX, PERIODIC, R, 2>(a, alpha);
+    if (a.stage == RK2_2 || a.stage == RK3_2 || a.stage == RK3_3)
+        return persistent_stage<K, FLUX, PERIODIC, R, 1>(a, alpha);
+    return persistent_stage<K, FLUX, PERIODIC, R, 0>(a, alpha);
+}
+
+#ifndef HRW_PF_WAVES_PER_EU
+#define HRW_PF_WAVES_PER_EU 3
+#endif
+
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_PF_WAVES_PER_EU)))
+void stage1d_pf_kernel(Stage1DArgs a)
+{
+    __shared__ double s_red[kBlock / 64 + 1];
+    const int lane = threadIdx.x & 63;
+    double alpha = a.alpha;
+    if constexpr (ALPHAMAX) {
+        alpha = wave_max(__longlong_as_double((long long)a.alpha_in[lane * kSlotStride]));
+        if (a.alpha_record && blockIdx.x == 0 && threadIdx.x == 0) *a.alpha_record = alpha;
+    }
+    clear_set(a.alpha_clear);
+    const double m_out = persistent_stage_any<K, FLUX, PERIODIC, R>(a, alpha);
+    if constexpr (ALPHAMAX) publish_max(m_out, a.alpha_out, s_red);
+}
+
 // One line: wave w of block b owns the chunk starting at W = (4b + w)*64R.
 template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_FAST_WAVES_PER_EU)))
@@ -321,6 +406,7 @@ void stage1d_fast_kernel(Stage1DArgs a)
     if constexpr (ALPHAMAX) {
         // every wave reduces the 64 partial maxima itself (no barrier)
         alpha = wave_max(__longlong_as_double((long long)a.alpha_in[lane * kSlotStride]));
+        if (a.alpha_record && blockIdx.x == 0 && threadIdx.x == 0) *a.alpha_record = alpha;
     }
     clear_set(a.alpha_clear);
     const double m_out = chunk_stage_any<K, FLUX, PERIODIC, R>(a, W, alpha);

@@ -21,8 +21,10 @@
 // from c0 = W + (l-1)*R; lanes 0 and 63 are halo lanes whose results are only used through
 // the neighbour exchange.  Each lane scatters the window cotangents of its R reconstructions
 // into R + 2(K-1) accumulators and hands the K-1 cells that spill past each end to the
-// neighbouring lane (ds_bpermute).  Anything the wave-edge lanes get wrong from a missing
-// neighbour stays within their own cells when R >= 2K-2, which R >= 4 guarantees.
+// neighbouring lane (ds_bpermute).  The wave-edge lanes load the window and flux-cotangent
+// cells beyond them directly (no neighbour to shuffle from), so their spills into lanes 1
+// and 62 are exact for any R >= 2; only their own cells (which miss the outer spills) are
+// dropped.
 // Positions outside [0, n) stand for their periodic images (every (cell, offset) pair of the
 // periodic stencil is counted once, also for lines shorter than the stencil).
 //
@@ -96,17 +98,33 @@ __device__ __forceinline__ int pending_cell(const Pending &p, int line)
 __device__ __forceinline__ double pending_delta(const Pending &p, int j, long long off, double h,
                                                 int first, int nc)
 {
+    // 16 independent loads in flight per lane per round (1024 partials), fixed order
     const int lane = threadIdx.x & 63;
-    double sum = 0.0;
-    for (int c = lane; c < nc; c += 64) sum += p.partial[first + c];
-    sum = __shfl(wave_sum(sum), 0, 64);
+    const double *q = p.partial + first;
+    double acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+    int c = lane;
+    for (; c + 15 * 64 < nc; c += 16 * 64) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u] += q[c + u * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+        if (c + u * 64 < nc) acc[u] += q[c + u * 64];
+#pragma unroll
+    for (int w = 8; w > 0; w >>= 1) {
+#pragma unroll
+        for (int u = 0; u < w; ++u) acc[u] += acc[u + w];
+    }
+    const double sum = __shfl(wave_sum(acc[0]), 0, 64);
     const double sj = p.s[off + j];
     return p.cf * h * sum * (double)((sj > 0.0) - (sj < 0.0));
 }
 
-// Add a pending correction to the values vals[i] a lane loaded from cells cell[i] (only where
-// `active`).  Must be called by the whole wave.
-template <int R>
+// Add a pending correction to the N values vals[i] a lane loaded from cells cell[i] (only
+// where `active`).  Must be called by the whole wave.
+template <int N>
 __device__ __forceinline__ void apply_pending(const Pending &p, int line, long long off, double h,
                                               int first, int nc, const long long *cell,
                                               bool active, double *vals)
@@ -115,11 +133,11 @@ __device__ __forceinline__ void apply_pending(const Pending &p, int line, long l
     if (j < 0) return;
     bool hit = false;
 #pragma unroll
-    for (int i = 0; i < R; ++i) hit = hit || (active && cell[i] == j);
+    for (int i = 0; i < N; ++i) hit = hit || (active && cell[i] == j);
     if (!__any(hit)) return;
     const double d = pending_delta(p, j, off, h, first, nc);
 #pragma unroll
-    for (int i = 0; i < R; ++i)
+    for (int i = 0; i < N; ++i)
         if (active && cell[i] == j) vals[i] += d;
 }
 
@@ -242,7 +260,7 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
 {
     constexpr int H = K - 1;
     constexpr int NV = R + 2 * H;
-    static_assert(R >= 4, "halo lanes need R >= 2K-2");
+    static_assert(R >= 2, "two or more cells per lane");
     const int lane = threadIdx.x & 63;
     const long long c0 = (long long)W + (long long)(lane - 1) * R;
     const bool owned = lane >= 1 && lane <= 62;
@@ -256,6 +274,15 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
         for (int i = 0; i < H; ++i) {
             v[i] = __shfl_up(v[R + i], 1, 64);
             v[H + R + i] = __shfl_down(v[H + i], 1, 64);
+        }
+        // the wave-edge lanes have no neighbour to shuffle from
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) v[i] = s[c0 - H + i];
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) v[H + R + i] = s[c0 + R + i];
         }
     } else {
 #pragma unroll
@@ -276,6 +303,21 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     }
     phi[0] = __shfl_up(phi[R], 1, 64);
     phi[R + 1] = __shfl_down(phi[1], 1, 64);
+    {
+        // lanes 0 and 63 form the phi just outside their cells themselves: with R < 2K-1 their
+        // reconstructions reach the owned cells of lanes 1 and 62
+        const bool edge = lane == 0 || lane == 63;
+        const long long pos = lane == 0 ? c0 - 1 : c0 + R;
+        long long hc[1] = {interior ? pos : cell_index(pos, n, true)};
+        double hv[1] = {edge ? lam[hc[0]] : 0.0};
+        apply_pending<1>(a.pend[0], line, off, h, first, nc, hc, edge, hv);
+        if (edge) {
+            const double w = linear_edge(xmin, rx, hc[0] + 1) - linear_edge(xmin, rx, hc[0]);
+            const double ph = hv[0] * rcp_nr(w);
+            if (lane == 0) phi[0] = ph;
+            else phi[R + 1] = ph;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = 0.0;
 
@@ -368,14 +410,10 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     return abar;
 }
 
-// The K = 3 reverse keeps ~220 VGPRs live under the compiler's schedule; two waves per SIMD
-// still overlap one wave's loads with the other's long FP64 stretch.
-#ifndef HRW_ADJ_WAVES_PER_EU
-#define HRW_ADJ_WAVES_PER_EU 2
-#endif
-
+// Occupancy targets that keep the K = 3 reverse free of spills: at R = 2 it fits 168 VGPRs
+// (3 waves per SIMD), at R = 4 it needs ~250 (2 waves).
 template <int R, bool ALPHAMAX>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_ADJ_WAVES_PER_EU)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 2 ? 3 : 2)))
 void adjoint_stage_kernel(AdjArgs a)
 {
     const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;

@@ -81,6 +81,7 @@ SIGNATURES = {
     "hrw_batch_adjoint_create": [P, L, I, C.POINTER(P)],
     "hrw_adjoint_destroy": [P],
     "hrw_adjoint_set_stream": [P, P],
+    "hrw_adjoint_set_cells_per_lane": [P, I],
     "hrw_adjoint_forward": [P, P, P],
     "hrw_adjoint_backward": [P, P, P],
     "hrw_adjoint_info": [P, PL, PL, PI, PL],

@@ -461,6 +461,11 @@ class Adjoint:
         if self.batch:
             self._system._leave(self._cur)
 
+    def set_cells_per_lane(self, r: int):
+        """Cells per lane of the reverse kernel (2 or 4); results agree to rounding."""
+        call("hrw_adjoint_set_cells_per_lane", self._h, int(r))
+        return self
+
     def info(self) -> dict:
         neq, ns, b = C.c_int64(), C.c_int64(), C.c_int64()
         ck = C.c_int()

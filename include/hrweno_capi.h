@@ -154,7 +154,9 @@ int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, d
                       double g2b, int arith, hrw_problem_t *out);
 /* Kernel choice for the 1D FAST form (no reference counterpart; used by the autotuner):
  * variant 0 = register-blocked kernel with cells_per_lane (4 or 8) consecutive cells per lane,
- * variant 1 = LDS-tile kernel.  Results agree within the FAST tolerance either way. */
+ * variant 1 = LDS-tile kernel, variant 2 = the register-blocked kernel run persistently with
+ * the next chunk's loads issued ahead of the current chunk's arithmetic.  Variants 0 and 2
+ * give identical results; variant 1 agrees within the FAST tolerance. */
 int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane);
 int hrw_problem_destroy(hrw_problem_t p);
 int hrw_problem_neq(hrw_problem_t p, int64_t *neq);
@@ -224,6 +226,8 @@ int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps, in
 int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_every, hrw_adjoint_t *out);
 int hrw_adjoint_destroy(hrw_adjoint_t a);
 int hrw_adjoint_set_stream(hrw_adjoint_t a, hrw_stream_t stream);
+/* Cells per lane of the reverse kernel (2 or 4, default 2); results agree to rounding. */
+int hrw_adjoint_set_cells_per_lane(hrw_adjoint_t a, int cells_per_lane);
 int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN);
 /* gN may equal g0; needs a preceding hrw_adjoint_forward (may be called again after it). */
 int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g0);

@@ -76,11 +76,11 @@ def test_tiling_invariance_exact(oracle, hh, dev, n):
 @pytest.mark.parametrize("bc", [0, 1])
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
-    """FAST form through both 1D kernels (register-blocked with 4 or 8 cells per lane, and the
-    LDS tile) on line lengths that hit every edge/interior case: each within the FAST
-    tolerance of the oracle, and the two register-blocked tilings bit-identical (the per-cell
-    arithmetic does not depend on which lane computes it)."""
-    for n in (1, 2, 5, 37, 255, 256, 2049, 2050, 4093, 20000):
+    """FAST form through the 1D kernels (register-blocked with 4 or 8 cells per lane, its
+    persistent prefetching form, and the LDS tile) on line lengths that hit every
+    edge/interior case: each within the FAST tolerance of the oracle, and the register-blocked
+    forms bit-identical (the per-cell arithmetic does not depend on which lane computes it)."""
+    for n in (1, 2, 5, 37, 255, 256, 2049, 2050, 4093, 20000, 300001):
         rng = np.random.default_rng(1000 * k + n)
         u0 = 0.4 + rng.standard_normal(n) * 0.3 + (np.arange(n) > n // 3)
         e = -1.0 + (2.0 / n) * np.arange(n + 1)
@@ -89,7 +89,7 @@ def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
         ou, ot, steps, fe = oracle.burgers_rk(u0, w, k=k, bc=bc, flux=0 if bc == 0 else 1,
                                               tout=5.5 * dt, dt=dt)
         res = {}
-        for variant, r in ((0, 8), (0, 4), (1, 8)):
+        for variant, r in ((0, 8), (0, 4), (2, 4), (1, 8)):
             p = hh.Burgers1D(n, k, 1e-6, flux=0 if bc == 0 else 1, bc=bc, arith="fast")
             p.tune(variant, r)
             ode = hh.RkTvd(p, order=3)
@@ -99,6 +99,7 @@ def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
             res[(variant, r)] = _h(u)
             assert_close(res[(variant, r)], ou)
         assert np.array_equal(res[(0, 8)], res[(0, 4)]), n
+        assert np.array_equal(res[(0, 4)], res[(2, 4)]), n
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
@@ -175,9 +176,12 @@ def test_fast_mstvd_and_rhs(oracle, hh, dev, r):
     ou, ot, fe = oracle.burgers_ms(c.u0, c.width, tout=tout, dt=c.dt)
     assert t == ot and ode.fevals == fe
     assert_close(_h(u), ou)
+    # L = -(F+ - F-)/w cancels ~1000x at this resolution, so a flux-level rounding difference
+    # is ~1e-12 of L itself on smooth cells: L is held to 1e-12 of max|L| (the state, above,
+    # also to 1e-12 relative per cell)
     L = _h(p.rhs(0.0, _t(c.u0, dev)))
-    ol, *_ = oracle.burgers_rk(c.u0, c.width, order=1, tout=0.5, dt=1.0, itask=2)
-    assert_close(c.u0 + L, ol)
+    oL = oracle.burgers_rhs(c.u0, c.width)
+    assert np.max(np.abs(L - oL)) <= 1e-12 * np.abs(oL).max()
 
 
 def test_mstvd_resume_across_calls(oracle, hh, dev):
