@@ -574,7 +574,7 @@ struct hrw_problem_s {
     double xmin = 0, rx = 0;
     double *d_width = nullptr;
     // 2D
-    int n1 = 0, n2 = 0, ty = 16;
+    int n1 = 0, n2 = 0, ty = 0;      // rows per wavefront, 0 = chosen at launch
     double x1min = 0, rx1 = 0, x2min = 0, rx2 = 0, g1a = 0, g1b = 0, g2a = 0, g2b = 0;
     // kernel choice for the 1D FAST form: 0 = register-blocked (cells_per_lane), 1 = LDS tile
     int variant = 0, cells_per_lane = 8;
@@ -676,6 +676,13 @@ extern "C" int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, doub
 extern "C" int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane)
 {
     if (!p) return fail(HRW_ERR_ARG, "null problem");
+    if (p->kind == PROB_PBE2D) {
+        if (variant != 0) return fail(HRW_ERR_ARG, "variant must be 0 for a 2D problem");
+        if (cells_per_lane != 0 && (cells_per_lane < 1 || cells_per_lane > 4096))
+            return fail(HRW_ERR_ARG, "rows per wavefront must be 0 (automatic) or 1..4096");
+        p->ty = cells_per_lane;
+        return HRW_OK;
+    }
     if (variant < 0 || variant > 2) return fail(HRW_ERR_ARG, "variant must be 0, 1 or 2");
     if (cells_per_lane != 4 && cells_per_lane != 8)
         return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
@@ -823,9 +830,22 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
     else l1d_x<3>(p->numflux, per, fast, amax, a, s);
 }
 
+// Rows per wavefront: the tuned value, else the fewest (at least 8) that let every wavefront
+// of the launch be resident at once.
 template <int K, int BC, bool FAST>
-static void l2d(const Stage2DArgs &a, hipStream_t s)
+static void l2d(Stage2DArgs a, hipStream_t s)
 {
+    if (a.ty <= 0) {
+        static int per_cu = 0;
+        if (!per_cu) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pbe2d_kernel<K, BC, FAST>, kBlock, 0) !=
+                    hipSuccess || per_cu <= 0)
+                per_cu = 4;
+        }
+        const long long slots = (long long)per_cu * device_cus() * (kBlock / 64);
+        const long long bands = std::max<long long>(1, slots / a.nstrips);
+        a.ty = (int)std::max<long long>(8, (a.n2 + bands - 1) / bands);
+    }
     const long long nsy = (a.n2 + a.ty - 1) / a.ty;
     const long long waves = nsy * a.nstrips;
     const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
@@ -1605,7 +1625,7 @@ struct hrw_adjoint_s {
     // output is applied by the launch that next reads that buffer (at most 3 launches later),
     // or by a fixup at the end of the backward.
     static constexpr int kSlots = 4;
-    double *partial[kSlots] = {};
+    double *partial[kSlots] = {}, *bpartial[kSlots] = {};
     unsigned long long *amax[kSlots] = {};
     unsigned tag = 0;
     int64_t launches = 0;
@@ -1627,6 +1647,7 @@ struct hrw_adjoint_s {
         }
         dfree(d_cline); dfree(d_cstart); dfree(d_lchunk0); dfree(d_lnch);
         for (auto &q : partial) dfree(q);
+        for (auto &q : bpartial) dfree(q);
         for (auto &q : amax) dfree(q);
     }
     Pending owed_by(const double *buf) const
@@ -1669,10 +1690,17 @@ static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
 {
     dfree(a->d_cline); dfree(a->d_cstart); dfree(a->d_lchunk0); dfree(a->d_lnch);
     for (auto &q : a->partial) dfree(q);
+    for (auto &q : a->bpartial) dfree(q);
     for (auto &q : a->amax) dfree(q);
     a->owed.clear();
     std::vector<int> cl, cs, c0, nc;
     for (int l = 0; l < (int)ncells.size(); ++l) {
+        // every line starts on a block of four chunks (idle chunks pad the gap), so its
+        // alpha-bar is summed in the same order as for a line on its own
+        while (cl.size() % 4) {
+            cl.push_back(-1);
+            cs.push_back(0);
+        }
         c0.push_back((int)cl.size());
         for (long long st = 0; st < ncells[l]; st += 62LL * a->R) {
             cl.push_back(l);
@@ -1688,6 +1716,7 @@ static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
     const size_t L = ncells.size();
     for (int q = 0; q < hrw_adjoint_s::kSlots; ++q) {
         CALL(dalloc(&a->partial[q], (size_t)a->nchunks));
+        CALL(dalloc(&a->bpartial[q], (size_t)(a->nchunks + 3) / 4));
         CALL(dalloc(&a->amax[q], L));
         fill_u64_kernel<<<grid_for((long long)L, 256, 4096), 256, 0, a->strm()>>>(a->amax[q], (long long)L);
     }
@@ -1919,6 +1948,7 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
         }
         const int slot = (int)(a->launches % hrw_adjoint_s::kSlots);
         g.partial_out = a->partial[slot];
+        g.bpartial_out = a->bpartial[slot];
         g.amax_out = a->amax[slot];
         g.tag = ++a->tag;
         g.pend[0] = a->owed_by(lam);
@@ -1926,6 +1956,7 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
         g.pend[2] = x2 ? a->owed_by(x2) : Pending{};
         Pending mine{};
         mine.partial = g.partial_out;
+        mine.bpartial = g.bpartial_out;
         mine.amax = g.amax_out;
         mine.s = g.s;
         mine.cf = cc;

@@ -5,13 +5,14 @@
 //
 // Work decomposition (no LDS, no barriers): one wavefront owns a strip of 64 consecutive
 // columns i = X0-1 .. X0+62 and marches over TY rows.  Lane l reconstructs column X0-1+l
-// along x1 (its 2K-1 window read through the caches, coalesced across lanes) and along x2
-// (a window sliding down the column in registers).  Interface fluxes along x1 pass between
-// neighbouring lanes by shuffles, so lanes 1..62 own output cells.
+// along x2 (a window sliding down the column in registers) and along x1 (its window gathered
+// from the neighbouring lanes' column values by lane shifts, plus K-1 halo values loaded by
+// the edge lanes).  Interface fluxes along x1 pass between neighbouring lanes the same way, so
+// lanes 1..62 own output cells.  Each cell value is read from HBM once per row band.
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "kernels_1d.hpp"
+#include "kernels_1d_fast.hpp"
 
 namespace hrw {
 
@@ -34,9 +35,58 @@ constexpr int kStrip = 62;
 template <int K, bool FAST>
 __device__ __forceinline__ void recon(const double *x, double eps, double &vl, double &vr)
 {
-    if constexpr (FAST) weno_cell_fast<K>(x, eps, vl, vr);
-    else weno_cell_exact<K>(x, eps, nullptr, vl, vr);
+    if constexpr (!FAST) {
+        weno_cell_exact<K>(x, eps, nullptr, vl, vr);
+    } else if constexpr (K == 3) {   // the 1D kernel's form (scaled triples, reciprocals)
+        recon3(12.0 * eps, tri3(x[0], x[1], x[2]), tri3(x[1], x[2], x[3]), tri3(x[2], x[3], x[4]), vl, vr);
+    } else if constexpr (K == 2) {
+        recon2(eps, pair2(x[0], x[1]), pair2(x[1], x[2]), vl, vr);
+    } else {
+        vl = vr = x[0];
+    }
 }
+
+// The x2 reconstruction marching down a column: the pieces of each triple (pair) of rows are
+// formed once and reused by the three (two) rows whose windows contain it.
+template <int K, bool FAST>
+struct ColumnRecon {
+    double cw[2 * K - 1];                // u(i, r-K+1 .. r+K-1) for the current row r
+    [[maybe_unused]] Tri3 t2, t1;        // K = 3, FAST: T(r-2), T(r-1)
+    [[maybe_unused]] Pair2 p1;           // K = 2, FAST: pair (r-1, r)
+
+    __device__ __forceinline__ void start(double eps)
+    {
+        (void)eps;
+        if constexpr (FAST && K == 3) {
+            t2 = tri3(cw[0], cw[1], cw[2]);
+            t1 = tri3(cw[1], cw[2], cw[3]);
+        } else if constexpr (FAST && K == 2) {
+            p1 = pair2(cw[0], cw[1]);
+        }
+    }
+    // reconstruction of the window now in cw (the triples of its lower part already formed)
+    __device__ __forceinline__ void recon_current(double eps, double &vl, double &vr)
+    {
+        if constexpr (FAST && K == 3) {
+            const Tri3 t0 = tri3(cw[2], cw[3], cw[4]);
+            recon3(12.0 * eps, t2, t1, t0, vl, vr);
+            t2 = t1;
+            t1 = t0;
+        } else if constexpr (FAST && K == 2) {
+            const Pair2 p0 = pair2(cw[1], cw[2]);
+            recon2(eps, p1, p0, vl, vr);
+            p1 = p0;
+        } else {
+            recon<K, FAST>(cw, eps, vl, vr);
+        }
+    }
+    __device__ __forceinline__ void push(double v)
+    {
+#pragma unroll
+        for (int m = 0; m < 2 * K - 2; ++m) cw[m] = cw[m + 1];
+        cw[2 * K - 2] = v;
+    }
+};
 
 // Godunov flux of the linear flux g*v at one interface (src/hrweno_fluxes.f90:66-74).
 __device__ __forceinline__ double godunov_lin(double g, double vm, double vp)
@@ -44,11 +94,58 @@ __device__ __forceinline__ double godunov_lin(double g, double vm, double vp)
     return godunov(g * vm, g * vp, vm, vp);
 }
 
+// Whole-wavefront shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l receives src of lane
+// l-1 (l+1); the lane with no source (0, resp. 63) receives its own `old`.  Every lane of the
+// wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double old, double src)
+{
+    const long long o = __builtin_bit_cast(long long, old), v = __builtin_bit_cast(long long, src);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_left(double old, double v) { return dpp_shift<0x138>(old, v); }
+__device__ __forceinline__ double from_right(double old, double v) { return dpp_shift<0x130>(old, v); }
+
+// Blocks are dispatched round-robin over the 8 XCDs.  This maps block b to a logical block so
+// that each XCD works on one contiguous range of logical blocks (the rows a band of
+// wavefronts shares with the bands above and below stay in that XCD's L2).  A bijection on
+// [0, nb).
+__device__ __forceinline__ unsigned xcd_major(unsigned b, unsigned nb)
+{
+    constexpr unsigned kXcd = 8;
+    const unsigned q = nb / kXcd, r = nb % kXcd, x = b % kXcd, loc = b / kXcd;
+    return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+}
+
+// The x1 window of row j, u(clamp(i+m), j) for m = -(K-1)..K-1, from the lanes' centre values
+// c = u(clamp(i), j) and the halo values: lanes 0..K-2 hold u(clamp(i-(K-1)), j) and lanes
+// 64-(K-1)..63 hold u(clamp(i+(K-1)), j).
+template <int K>
+__device__ __forceinline__ void x1_window(double c, double halo, double *x)
+{
+    x[K - 1] = c;
+    if constexpr (K == 2) {
+        x[0] = from_left(halo, c);
+        x[2] = from_right(halo, c);
+    } else if constexpr (K == 3) {
+        x[1] = from_left(from_right(0.0, halo), c);    // lane 0: lane 1's halo
+        x[0] = from_left(halo, x[1]);                  // lane 0: its own halo
+        x[3] = from_right(from_left(0.0, halo), c);    // lane 63: lane 62's halo
+        x[4] = from_right(halo, x[3]);                 // lane 63: its own halo
+    }
+}
+
+// Work decomposition: one wavefront owns a strip of 64 columns and TY rows; the loads of row
+// j+1 (the new x2 window value, the x1 halo, the stage operands) are issued before row j is
+// computed.
 template <int K, int BC, bool FAST>
 __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const unsigned blk = xcd_major(blockIdx.x, gridDim.x);
+    const long long wave = (long long)blk * (kBlock / 64) + (threadIdx.x >> 6);
     const int sx = (int)(wave % a.nstrips);
     const long long sy = wave / a.nstrips;
     const int n1 = a.n1, n2 = a.n2;
@@ -56,35 +153,34 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
     if (Y0 >= n2) return;                               // whole wavefront leaves together
     const int Y1 = (int)min((long long)n2, Y0 + a.ty);
     const int i = sx * kStrip - 1 + lane;               // column of this lane (may be -1 or >= n1)
-    const int ic = i < 0 ? 0 : (i >= n1 ? n1 - 1 : i);  // clamped column for loads
+    auto colc = [n1](int c) { return c < 0 ? 0 : (c >= n1 ? n1 - 1 : c); };
+    const int ic = colc(i);                             // clamped column for loads
     const bool own = lane >= 1 && lane <= kStrip && i < n1;
+    const bool has_halo = lane < K - 1 || lane >= 64 - (K - 1);
+    const int hc = colc(lane < 32 ? i - (K - 1) : i + (K - 1));
 
     // per-lane constants: x1 width, g1 at the right interface x_{i+1/2}
     const double e_l = linear_edge(a.x1min, a.rx1, ic), e_r = linear_edge(a.x1min, a.rx1, ic + 1);
     const double w1 = e_r - e_l;
     const double g1 = linear_coef(a.g1a, a.g1b, e_r);
-    int xi[2 * K - 1];                                  // clamped x1 window columns
-#pragma unroll
-    for (int m = -(K - 1); m <= K - 1; ++m) {
-        int c = i + m;
-        xi[m + K - 1] = c < 0 ? 0 : (c >= n1 ? n1 - 1 : c);
-    }
     auto rowc = [n2](long long j) -> long long { return j < 0 ? 0 : (j >= n2 ? n2 - 1 : j); };
     const double *us = a.us;
+    const int stage = a.stage;
+    const bool need_u0 = stage == RK2_2 || stage == RK3_2 || stage == RK3_3;
+    const bool need_ms = stage == MS3;
 
     // column window for the x2 reconstruction of row r: cw[m] = u(i, r-K+1+m)
-    double cw[2 * K - 1];
+    ColumnRecon<K, FAST> col;
     // prologue: reconstruction of row Y0-1 (for F2 at Y0-1/2) and of row Y0
     double vl2c, vr2c, f2_dn;
     {
 #pragma unroll
-        for (int m = 0; m < 2 * K - 1; ++m) cw[m] = us[rowc(Y0 - 1 - (K - 1) + m) * n1 + ic];
+        for (int m = 0; m < 2 * K - 1; ++m) col.cw[m] = us[rowc(Y0 - 1 - (K - 1) + m) * n1 + ic];
+        col.start(a.eps);
         double vlp, vrp;
-        recon<K, FAST>(cw, a.eps, vlp, vrp);
-#pragma unroll
-        for (int m = 0; m < 2 * K - 2; ++m) cw[m] = cw[m + 1];
-        cw[2 * K - 2] = us[rowc(Y0 + K - 1) * n1 + ic];
-        recon<K, FAST>(cw, a.eps, vl2c, vr2c);
+        col.recon_current(a.eps, vlp, vrp);
+        col.push(us[rowc(Y0 + K - 1) * n1 + ic]);
+        col.recon_current(a.eps, vl2c, vr2c);
         if (Y0 == 0) {
             f2_dn = 0.0;                                // zero flux at x2 = x2min
         } else {
@@ -93,13 +189,40 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
         }
     }
 
+    // operands of row j, loaded one row ahead
+    double nx_push = us[rowc(Y0 + K) * n1 + ic];
+    double nx_halo = has_halo ? us[Y0 * n1 + hc] : 0.0;
+    double nx_z = 0.0, nx_u4 = 0.0, nx_l4 = 0.0;
+    if (own) {
+        const long long g = Y0 * n1 + i;
+        if (need_u0) nx_z = a.u0[g];
+        if (need_ms) {
+            nx_u4 = a.uo4[g];
+            nx_l4 = a.Lo4[g];
+        }
+    }
+
+    const double iw1 = FAST ? rcp_nr(w1) : 0.0;
     for (int j = (int)Y0; j < Y1; ++j) {
+        const double push = nx_push, halo = nx_halo, z0 = nx_z, u4 = nx_u4, l4 = nx_l4;
+        if (j + 1 < Y1) {
+            nx_push = us[rowc((long long)j + 1 + K) * n1 + ic];
+            if (has_halo) nx_halo = us[(long long)(j + 1) * n1 + hc];
+            if (own) {
+                const long long g = (long long)(j + 1) * n1 + i;
+                if (need_u0) nx_z = a.u0[g];
+                if (need_ms) {
+                    nx_u4 = a.uo4[g];
+                    nx_l4 = a.Lo4[g];
+                }
+            }
+        }
+        const double uc = col.cw[K - 1];                // u(i, j)
+
         // x2: reconstruction of row j+1, flux at j+1/2
-#pragma unroll
-        for (int m = 0; m < 2 * K - 2; ++m) cw[m] = cw[m + 1];
-        cw[2 * K - 2] = us[rowc((long long)j + K) * n1 + ic];
+        col.push(push);
         double vl2n, vr2n;
-        recon<K, FAST>(cw, a.eps, vl2n, vr2n);
+        col.recon_current(a.eps, vl2n, vr2n);
         const double e2l = linear_edge(a.x2min, a.rx2, j), e2r = linear_edge(a.x2min, a.rx2, j + 1);
         const double w2 = e2r - e2l;
         double f2_up;
@@ -110,36 +233,28 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
         }
 
         // x1: reconstruction of (i, j), flux at i+1/2 with the right neighbour's vl
-        const double *row = us + (long long)j * n1;
         double xw[2 * K - 1];
-#pragma unroll
-        for (int m = 0; m < 2 * K - 1; ++m) xw[m] = row[xi[m]];
+        x1_window<K>(uc, halo, xw);
         double vl1, vr1;
         recon<K, FAST>(xw, a.eps, vl1, vr1);
-        const double vl1_right = __shfl_down(vl1, 1, 64);
+        const double vl1_right = from_right(vl1, vl1);
         double f1_r;
         if (i < 0) f1_r = 0.0;                           // zero flux at x1 = x1min
         else if (i >= n1 - 1) f1_r = BC == 0 ? 0.0 : godunov_lin(g1, vr1, vr1);
         else f1_r = godunov_lin(g1, vr1, vl1_right);
-        const double f1_l = __shfl_up(f1_r, 1, 64);
+        const double f1_l = from_left(f1_r, f1_r);
 
         if (own) {
             const long long g = (long long)j * n1 + i;
             // example2:124-126
             double L;
-            if constexpr (FAST) L = (f1_l - f1_r) / w1 - (f2_up - f2_dn) / w2;
+            if constexpr (FAST) L = fma_(f1_l - f1_r, iw1, -((f2_up - f2_dn) * rcp_nr(w2)));
             else L = -((f1_r - f1_l) / w1) - (f2_up - f2_dn) / w2;
-            const double uc = xw[K - 1];
-            if (a.stage < 0) {
+            if (stage < 0) {
                 a.Lout[g] = L;
             } else {
-                double z = uc, u4 = 0.0, l4 = 0.0;
-                if (a.stage == RK2_2 || a.stage == RK3_2 || a.stage == RK3_3) z = a.u0[g];
-                if (a.stage == MS3) {
-                    u4 = a.uo4[g];
-                    l4 = a.Lo4[g];
-                }
-                const double un = stage_update<FAST>(a.stage, z, uc, L, a.h, u4, l4);
+                const double z = need_u0 ? z0 : uc;
+                const double un = stage_update<FAST>(stage, z, uc, L, a.h, u4, l4);
                 if (a.Lout) a.Lout[g] = L;
                 a.uo[g] = un;
             }

@@ -17,7 +17,7 @@
 // first cell attaining the maximum) alpha-bar lands on that cell with the sign of us there.
 //
 // Layout.  Wavefront c of the grid takes chunk c of a chunk table: OUTPUT cells
-// [W, W + 62R) of line `line` (W = start[c]).  Lane l (0..63) owns the R consecutive cells
+// [W, W + 62R) of line `line` (W = start[c]; line < 0 marks an idle padding chunk).  Lane l (0..63) owns the R consecutive cells
 // from c0 = W + (l-1)*R; lanes 0 and 63 are halo lanes whose results are only used through
 // the neighbour exchange.  Each lane scatters the window cotangents of its R reconstructions
 // into R + 2(K-1) accumulators and hands the K-1 cells that spill past each end to the
@@ -30,7 +30,8 @@
 //
 // alpha-bar is a reduction over a whole line, so its correction to the output (at the argmax
 // cell) is known only when the launch has finished.  Each wave writes its partial sum to
-// partial_out[c] and, if it holds the argmax, atomicMax-es the key (tag << 32 | ~cell) into
+// partial_out[c] (and each block the sum of its four, in order, to bpartial_out[b]) and, if it
+// holds the argmax, atomicMax-es the key (tag << 32 | ~cell) into
 // amax_out[line] (tags grow with every launch, so no slot is ever reset and the smallest cell
 // wins).  The launch that next reads this output (as lam, x1 or x2) gets it as a Pending and
 // adds the correction  cf*h * sum(partials) * sign(s_prev[cell])  to the value it loads at
@@ -59,6 +60,7 @@ __device__ __forceinline__ unsigned wave_umin(unsigned v)
 // The not yet applied alpha correction of an earlier launch's output.
 struct Pending {
     const double *partial;              // that launch's per-chunk partial sums (null: none)
+    const double *bpartial;             // ... and per-block sums of its four chunks
     const unsigned long long *amax;     // its per-line argmax keys
     const double *s;                    // its stage input (for the sign at the argmax)
     double cf;                          // its c (the correction is cf * h[line] * sum)
@@ -80,14 +82,13 @@ struct AdjArgs {
     const unsigned long long *aset;   // alpha partials: line l, partial p at
     int aline_stride, anpart, apart_stride;   //   aset[l*aline_stride + p*apart_stride]
     double *partial_out;        // this launch: one partial alpha-bar per chunk
+    double *bpartial_out;       // this launch: their sums per block of four chunks
     unsigned long long *amax_out;     // this launch: argmax key per line
     unsigned tag;
     Pending pend[3];            // corrections owed by lam, x1, x2
 };
 
-// Resolve a pending correction for `line`: the cell j and the amount delta.  Wave-uniform.
-// `wanted` (per lane) says whether this lane holds cell j; the partial sum is only formed
-// when some lane does.
+// The cell a pending correction applies to on `line` (wave-uniform), or -1 if none.
 __device__ __forceinline__ int pending_cell(const Pending &p, int line)
 {
     if (!p.partial) return -1;
@@ -95,29 +96,46 @@ __device__ __forceinline__ int pending_cell(const Pending &p, int line)
     if ((unsigned)(key >> 32) != p.tag) return -1;
     return (int)(0xffffffffu - (unsigned)(key & 0xffffffffull));
 }
-__device__ __forceinline__ double pending_delta(const Pending &p, int j, long long off, double h,
-                                                int first, int nc)
+// Sum of q[0..count) by one wave in a fixed order: 16 independent loads in flight per lane
+// per round (1024 values per round).
+__device__ __forceinline__ double wave_sum_array(const double *q, int count)
 {
-    // 16 independent loads in flight per lane per round (1024 partials), fixed order
     const int lane = threadIdx.x & 63;
-    const double *q = p.partial + first;
     double acc[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] = 0.0;
     int c = lane;
-    for (; c + 15 * 64 < nc; c += 16 * 64) {
+    for (; c + 15 * 64 < count; c += 16 * 64) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) acc[u] += q[c + u * 64];
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-        if (c + u * 64 < nc) acc[u] += q[c + u * 64];
+        if (c + u * 64 < count) acc[u] += q[c + u * 64];
 #pragma unroll
     for (int w = 8; w > 0; w >>= 1) {
 #pragma unroll
         for (int u = 0; u < w; ++u) acc[u] += acc[u + w];
     }
-    const double sum = __shfl(wave_sum(acc[0]), 0, 64);
+    return __shfl(wave_sum(acc[0]), 0, 64);
+}
+
+// The line's alpha-bar: the chunks [first, first + nc) are summed as the chunks before the
+// first whole block of four, the whole blocks' sums, and the chunks after the last one.
+__device__ __forceinline__ double pending_delta(const Pending &p, int j, long long off, double h,
+                                                int first, int nc)
+{
+    const int end = first + nc;
+    const int b0 = (first + 3) >> 2, b1 = end >> 2;   // whole blocks [b0, b1)
+    double sum;
+    if (b1 <= b0) {
+        sum = wave_sum_array(p.partial + first, nc);
+    } else {
+        double head = 0.0, tail = 0.0;
+        for (int c = first; c < 4 * b0; ++c) head += p.partial[c];
+        for (int c = 4 * b1; c < end; ++c) tail += p.partial[c];
+        sum = (head + wave_sum_array(p.bpartial + b0, b1 - b0)) + tail;
+    }
     const double sj = p.s[off + j];
     return p.cf * h * sum * (double)((sj > 0.0) - (sj < 0.0));
 }
@@ -418,7 +436,14 @@ void adjoint_stage_kernel(AdjArgs a)
 {
     const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wave >= a.nchunks) return;   // whole wavefronts leave together
+    __shared__ double s_part[kBlock / 64];
+    if (wave >= a.nchunks || a.chunk_line[wave] < 0) {   // idle chunks only join the block sum
+        if constexpr (ALPHAMAX) {
+            if (lane == 0) s_part[threadIdx.x >> 6] = 0.0;
+            __syncthreads();
+        }
+        return;
+    }
     const int line = a.chunk_line[wave];
     const long long off = a.off[line];
     const int n = a.n[line];
@@ -457,10 +482,14 @@ void adjoint_stage_kernel(AdjArgs a)
         const unsigned wmin = wave_umin(cand);
         if (lane == 0) {
             a.partial_out[wave] = wsum;
+            s_part[threadIdx.x >> 6] = wsum;
             if (wmin != 0xffffffffu)
                 atomicMax(a.amax_out + line,
                           ((unsigned long long)a.tag << 32) | (unsigned long long)(0xffffffffu - wmin));
         }
+        __syncthreads();
+        // used only for blocks whose four chunks lie on one line
+        if (threadIdx.x == 0) a.bpartial_out[blockIdx.x] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
     }
 }
 

@@ -156,7 +156,10 @@ int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, d
  * variant 0 = register-blocked kernel with cells_per_lane (4 or 8) consecutive cells per lane,
  * variant 1 = LDS-tile kernel, variant 2 = the register-blocked kernel run persistently with
  * the next chunk's loads issued ahead of the current chunk's arithmetic.  Variants 0 and 2
- * give identical results; variant 1 agrees within the FAST tolerance. */
+ * give identical results; variant 1 agrees within the FAST tolerance.  For a 2D problem
+ * variant must be 0 and cells_per_lane is the number of rows each wavefront marches over
+ * (0 = chosen at launch so that every wavefront is resident at once); results are identical
+ * for every value. */
 int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane);
 int hrw_problem_destroy(hrw_problem_t p);
 int hrw_problem_neq(hrw_problem_t p, int64_t *neq);

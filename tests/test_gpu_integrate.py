@@ -301,6 +301,35 @@ def test_pbe_fast_within_tolerance_and_mass(oracle, hh, dev):
     assert abs(mass(g) - mass(u0)) <= 1e-12 * mass(u0)
 
 
+@pytest.mark.parametrize("n1,n2", [(1, 1), (2, 3), (5, 4), (63, 2), (125, 9)])
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_pbe_small_and_ragged_exact_bitwise(oracle, hh, dev, n1, n2, k):
+    """Grids narrower than the x1 window, one row, a strip plus one column: the lane-shifted
+    x1 windows clamp at the edges exactly as the reference's ghost-free stencils do."""
+    u0, e1, e2, w1, w2, dt = pbe_case(n1, n2, seed=11)
+    ode = hh.MsTvd(hh.PBE2D(n1, n2, k=k, bc=3, arith="exact"))
+    u = _t(u0, dev)
+    t = ode.integrate(u, 0.0, 7.5 * dt, dt)
+    ou, ot, fe = oracle.pbe_ms(u0, e1, e2, k=k, bc=1, tout=7.5 * dt, dt=dt)
+    assert t == ot and ode.fevals == fe
+    assert np.array_equal(_h(u), ou)
+
+
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+def test_pbe_rows_per_wave_identical(hh, dev, arith):
+    """The rows each wavefront marches over (tuned or automatic) do not change any result."""
+    n1, n2 = 131, 70
+    u0, e1, e2, w1, w2, dt = pbe_case(n1, n2, seed=9)
+    res = []
+    for ty in (0, 1, 5, 16, 64, 100):
+        ode = hh.MsTvd(hh.PBE2D(n1, n2, bc=3, arith=arith).tune(0, ty))
+        u = _t(u0, dev)
+        ode.integrate(u, 0.0, 10.5 * dt, dt)
+        res.append(_h(u))
+    for r in res[1:]:
+        assert np.array_equal(r, res[0])
+
+
 def test_config4_full_size_exact_steps(oracle, hh, dev):
     """2048 x 2048: 4 starting RK3 steps + 2 multistep steps bit-identical to the oracle."""
     from hrweno_hip import workloads as W
