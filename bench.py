@@ -132,7 +132,10 @@ def run_plain(args, torch, hh, W, dev):
         raise RuntimeError("non-finite state after the timed steps")
     log(f"timed {args.steps} steps: {ms:.3f} ms")
     fast_v2 = args.arith == "fast" and args.variant == 0
-    kernel = ("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%d>" % args.cells_per_lane
+    cpl = (str(args.cells_per_lane) if args.cells_per_lane
+           else "R,NPW> with <R,NPW> = <8,0> for stage 1 and <4,1> for stages 2-3")
+    kernel = ("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%s" % cpl
+              + (">" if args.cells_per_lane else "")
               if fast_v2 else "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith)
     rl = roofline(64.0 / 3 * c.n, ms / (3 * args.steps), kernel, "stage1d_2p24")
     config = {"workload": "burgers1d_weno5_lf_rk3_periodic_2^24cells", "cells": c.n, "k": 3,
@@ -201,8 +204,10 @@ def run_pbe(args, torch, hh, W, dev):
 
 
 def run_adjoint(args, torch, hh, W, dev):
-    """Gradient of 0.5*|u_N - target|^2 w.r.t. u0 for 2**20 cells x 400 RK3 steps, checkpoints
-    every 20 steps: the recording forward + the backward (recompute + reverse kernels)."""
+    """Gradient of 0.5*|u_N - target|^2 w.r.t. u0 for 2**20 cells x 400 RK3 steps: the
+    recording forward + the backward.  The default checkpoint interval is automatic (the whole
+    tape fits in HBM, so the backward is the reverse kernels alone); the low-memory schedule
+    (a checkpoint every 20 steps, segments recomputed) is timed beside it."""
     c, noise = W.config5(steps=args.steps)
     n, steps = c.n, c.steps
     prob = hh.Burgers1D(n)
@@ -225,9 +230,17 @@ def run_adjoint(args, torch, hh, W, dev):
     full.forward(u0)
     full.backward(gN)
     ms_r = timed(torch, lambda: full.backward(gN))
-    info = adj.info()
+    del full
+    # the low-memory schedule: a checkpoint every 20 steps, each segment recomputed once
+    low = hh.Adjoint(prob, nsteps=steps, dt=c.dt, ckpt_every=min(20, steps))
+    low.backward(low.forward(u0) - target)
+    ms_lf = timed(torch, lambda: low.forward(u0))
+    ms_lb = timed(torch, lambda: low.backward(gN))
+    info, linfo = adj.info(), low.info()
+    del low
     log(f"adjoint: forward(recording) {ms_f:.2f} ms, backward {ms_b:.2f} ms, plain forward "
-        f"{ms_p:.2f} ms, reverse-only {ms_r:.2f} ms")
+        f"{ms_p:.2f} ms, reverse-only {ms_r:.2f} ms; checkpoint every 20: forward "
+        f"{ms_lf:.2f} ms, backward {ms_lb:.2f} ms")
     rl = roofline(88.0 / 3 * n, ms_r / (3 * steps), "hrw::adjoint_stage_kernel<2,alphamax>",
                   "adjoint_2p20")
     config = {"workload": "burgers1d_weno5_lf_rk3_adjoint_2^20cells_400steps", "cells": n,
@@ -238,7 +251,9 @@ def run_adjoint(args, torch, hh, W, dev):
               "gradient_over_forward": (ms_f + ms_b) / ms_p,
               "backward_over_forward": ms_b / ms_p,
               "reverse_only_over_forward": ms_r / ms_p,
-              "device_bytes": info["device_bytes"]}
+              "device_bytes": info["device_bytes"],
+              "ckpt20_gradient_over_forward": (ms_lf + ms_lb) / ms_p,
+              "ckpt20_device_bytes": linfo["device_bytes"]}
     return n * steps / ((ms_f + ms_b) / 1e3), (ms_f + ms_b) / steps, rl, config, \
         "synthetic (seeded SplitMix64, config-1 style data; target from N(0,0.01) noise)"
 
@@ -280,7 +295,7 @@ def run_adjoint_batch(args, torch, hh, W, dev):
                   "adjoint_batch")
     config = {"workload": "burgers1d_batch_adjoint_256_lines_log_uniform_1024_16384",
               "lines": len(cases), "cells": total, "steps": steps, "k": 3,
-              "ckpt_every": args.ckpt_every, "parallelism": "single GPU",
+              "ckpt_every": adj.info()["ckpt_every"], "parallelism": "single GPU",
               "value_is": "cell-steps of the whole gradient (recording forward + backward) per s",
               "per_line_cell_updates_per_s": pl_value, "per_line_steps": pl_steps,
               "speedup_vs_per_line": value / pl_value}
@@ -294,14 +309,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["plain", "batch", "pbe", "adjoint", "adjoint_batch"],
                     default="plain")
-    ap.add_argument("--ckpt-every", type=int, default=20)
+    ap.add_argument("--ckpt-every", type=int, default=0,
+                    help="adjoint checkpoint interval (0: the largest that fits in half of HBM)")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cells", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0,
                     help="1D FAST kernel: 0 register-blocked, 1 LDS tile")
-    ap.add_argument("--cells-per-lane", type=int, default=8)
+    ap.add_argument("--cells-per-lane", type=int, default=0,
+                    help="cells per lane of the 1D stage kernel (0: by stage)")
     ap.add_argument("--per-line-steps", type=int, default=5)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of --cells doubles (20 times), the "

@@ -22,6 +22,7 @@
 #include "kernels_1d_fast.hpp"
 #include "kernels_2d.hpp"
 #include "kernels_adjoint.hpp"
+#include "launch.hpp"
 
 using namespace hrw;
 
@@ -576,8 +577,9 @@ struct hrw_problem_s {
     // 2D
     int n1 = 0, n2 = 0, ty = 0;      // rows per wavefront, 0 = chosen at launch
     double x1min = 0, rx1 = 0, x2min = 0, rx2 = 0, g1a = 0, g1b = 0, g2a = 0, g2b = 0;
-    // kernel choice for the 1D FAST form: 0 = register-blocked (cells_per_lane), 1 = LDS tile
-    int variant = 0, cells_per_lane = 8;
+    // kernel choice for the 1D FAST form: 0 = register-blocked (cells_per_lane, 0 = by stage),
+    // 1 = LDS tile, 2 = register-blocked persistent
+    int variant = 0, cells_per_lane = 0;
     // alpha slot sets for rhs-only calls
     unsigned long long *slots = nullptr;
     bool alphamax() const { return kind == PROB_BURGERS1D && numflux == HRW_FLUX_LAX_FRIEDRICHS && alpha < 0; }
@@ -684,8 +686,8 @@ extern "C" int hrw_problem_tune(hrw_problem_t p, int variant, int cells_per_lane
         return HRW_OK;
     }
     if (variant < 0 || variant > 2) return fail(HRW_ERR_ARG, "variant must be 0, 1 or 2");
-    if (cells_per_lane != 4 && cells_per_lane != 8)
-        return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
+    if (cells_per_lane != 0 && cells_per_lane != 4 && cells_per_lane != 6 && cells_per_lane != 8)
+        return fail(HRW_ERR_ARG, "cells_per_lane must be 0 (automatic), 4, 6 or 8");
     p->variant = variant;
     p->cells_per_lane = cells_per_lane;
     return HRW_OK;
@@ -704,109 +706,6 @@ extern "C" int hrw_problem_neq(hrw_problem_t p, int64_t *neq)
 }
 
 // ---- kernel dispatch ----------------------------------------------------------------
-constexpr int kCPT = 4;   // cells per thread in the 1D stage kernel
-
-template <int K, int FLUX, bool PER, bool FAST, bool AMAX>
-static void l1d(const Stage1DArgs &a, hipStream_t s)
-{
-    constexpr int TILE = kBlock * kCPT - 2;
-    const unsigned grid = (unsigned)((a.n + TILE - 1) / TILE);
-    stage1d_kernel<K, FLUX, PER, FAST, AMAX, kCPT><<<grid, kBlock, 0, s>>>(a);
-}
-template <int K, int FLUX, bool PER, bool FAST>
-static void l1d_a(bool amax, const Stage1DArgs &a, hipStream_t s)
-{
-    if constexpr (FLUX == LF) {
-        if (amax) return l1d<K, FLUX, PER, FAST, true>(a, s);
-    }
-    l1d<K, FLUX, PER, FAST, false>(a, s);
-}
-template <int K, int FLUX, bool PER>
-static void l1d_f(bool fast, bool amax, const Stage1DArgs &a, hipStream_t s)
-{
-    if (fast) l1d_a<K, FLUX, PER, true>(amax, a, s);
-    else l1d_a<K, FLUX, PER, false>(amax, a, s);
-}
-template <int K, int FLUX>
-static void l1d_p(bool per, bool fast, bool amax, const Stage1DArgs &a, hipStream_t s)
-{
-    if (per) l1d_f<K, FLUX, true>(fast, amax, a, s);
-    else l1d_f<K, FLUX, false>(fast, amax, a, s);
-}
-template <int K>
-static void l1d_x(int flux, bool per, bool fast, bool amax, const Stage1DArgs &a, hipStream_t s)
-{
-    if (flux == LF) l1d_p<K, LF>(per, fast, amax, a, s);
-    else l1d_p<K, GODUNOV>(per, fast, amax, a, s);
-}
-template <int K, int FLUX, bool PER, bool AMAX, int R>
-static void l1f(const Stage1DArgs &a, hipStream_t s)
-{
-    constexpr long long per_block = 64LL * R * (kBlock / 64);
-    const unsigned grid = (unsigned)((a.n + per_block - 1) / per_block);
-    stage1d_fast_kernel<K, FLUX, PER, AMAX, R><<<grid, kBlock, 0, s>>>(a);
-}
-// Persistent form: as many blocks as fit on the device at once (occupancy of this
-// instantiation, asked once), never more than there are chunks.
-static int device_cus()
-{
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
-    return cus;
-}
-template <int K, int FLUX, bool PER, bool AMAX, int R>
-static void l1p(const Stage1DArgs &a, hipStream_t s)
-{
-    static int per_cu = 0;
-    if (!per_cu) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &nb, reinterpret_cast<const void *>(&stage1d_pf_kernel<K, FLUX, PER, AMAX, R>), kBlock, 0) != hipSuccess ||
-            nb <= 0)
-            nb = 2;
-        per_cu = nb;
-    }
-    const long long chunks = (a.n + 64LL * R - 1) / (64LL * R);
-    const long long want = (chunks + kBlock / 64 - 1) / (kBlock / 64);
-    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(want, (long long)per_cu * device_cus()));
-    stage1d_pf_kernel<K, FLUX, PER, AMAX, R><<<grid, kBlock, 0, s>>>(a);
-}
-template <int K, int FLUX, bool PER, bool AMAX>
-static void l1f_r(int r, const Stage1DArgs &a, hipStream_t s)
-{
-    if (r < 0) {
-        if (r == -4) l1p<K, FLUX, PER, AMAX, 4>(a, s);
-        else l1p<K, FLUX, PER, AMAX, 8>(a, s);
-        return;
-    }
-    if (r == 4) l1f<K, FLUX, PER, AMAX, 4>(a, s);
-    else l1f<K, FLUX, PER, AMAX, 8>(a, s);
-}
-template <int K, int FLUX, bool PER>
-static void l1f_a(bool amax, int r, const Stage1DArgs &a, hipStream_t s)
-{
-    if constexpr (FLUX == LF) {
-        if (amax) return l1f_r<K, FLUX, PER, true>(r, a, s);
-    }
-    l1f_r<K, FLUX, PER, false>(r, a, s);
-}
-template <int K>
-static void l1f_x(int flux, bool per, bool amax, int r, const Stage1DArgs &a, hipStream_t s)
-{
-    if (flux == LF) {
-        if (per) l1f_a<K, LF, true>(amax, r, a, s);
-        else l1f_a<K, LF, false>(amax, r, a, s);
-    } else {
-        if (per) l1f_a<K, GODUNOV, true>(amax, r, a, s);
-        else l1f_a<K, GODUNOV, false>(amax, r, a, s);
-    }
-}
-
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
@@ -817,57 +716,23 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
         const bool per = p->bc == HRW_BC_PERIODIC;
-        const int r = p->variant == 2 ? -p->cells_per_lane : p->cells_per_lane;   // < 0: persistent
-        if (p->k == 1) l1f_x<1>(p->numflux, per, amax0, r, a, s);
-        else if (p->k == 2) l1f_x<2>(p->numflux, per, amax0, r, a, s);
-        else l1f_x<3>(p->numflux, per, amax0, r, a, s);
+        // cells per lane: tuned, or by the stage's operand count (a stage that reads u0 as well
+        // is load-bound, so smaller chunks keep more loads in flight per compute unit)
+        const int cpl = p->cells_per_lane ? p->cells_per_lane : (stage_npw(a.stage) == 0 ? 8 : 4);
+        const int r = p->variant == 2 ? -cpl : cpl;   // < 0: persistent
+        if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, r, a, s);
+        else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, r, a, s);
+        else launch_fast_1d_k3(p->numflux, per, amax0, r, a, s);
         return;
     }
     const bool per = p->bc == HRW_BC_PERIODIC, fast = p->arith == HRW_ARITH_FAST;
     const bool amax = p->alphamax();
-    if (p->k == 1) l1d_x<1>(p->numflux, per, fast, amax, a, s);
-    else if (p->k == 2) l1d_x<2>(p->numflux, per, fast, amax, a, s);
-    else l1d_x<3>(p->numflux, per, fast, amax, a, s);
+    launch_tile_1d(p->k, p->numflux, per, fast, amax, a, s);
 }
 
-// Rows per wavefront: the tuned value, else the fewest (at least 8) that let every wavefront
-// of the launch be resident at once.
-template <int K, int BC, bool FAST>
-static void l2d(Stage2DArgs a, hipStream_t s)
-{
-    if (a.ty <= 0) {
-        static int per_cu = 0;
-        if (!per_cu) {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pbe2d_kernel<K, BC, FAST>, kBlock, 0) !=
-                    hipSuccess || per_cu <= 0)
-                per_cu = 4;
-        }
-        const long long slots = (long long)per_cu * device_cus() * (kBlock / 64);
-        const long long bands = std::max<long long>(1, slots / a.nstrips);
-        a.ty = (int)std::max<long long>(8, (a.n2 + bands - 1) / bands);
-    }
-    const long long nsy = (a.n2 + a.ty - 1) / a.ty;
-    const long long waves = nsy * a.nstrips;
-    const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
-    pbe2d_kernel<K, BC, FAST><<<grid, kBlock, 0, s>>>(a);
-}
-template <int K>
-static void l2d_b(int bc, bool fast, const Stage2DArgs &a, hipStream_t s)
-{
-    if (bc == HRW_BC_ZERO_FLUX) {
-        if (fast) l2d<K, 0, true>(a, s);
-        else l2d<K, 0, false>(a, s);
-    } else {
-        if (fast) l2d<K, 1, true>(a, s);
-        else l2d<K, 1, false>(a, s);
-    }
-}
 static void launch_stage2d(const hrw_problem_s *p, const Stage2DArgs &a, hipStream_t s)
 {
-    const bool fast = p->arith == HRW_ARITH_FAST;
-    if (p->k == 1) l2d_b<1>(p->bc, fast, a, s);
-    else if (p->k == 2) l2d_b<2>(p->bc, fast, a, s);
-    else l2d_b<3>(p->bc, fast, a, s);
+    launch_pbe2d(p->k, p->bc, p->arith == HRW_ARITH_FAST, a, s);
 }
 
 // One fused pass: out = stage(us, u0, L(us)) (and/or Lout = L(us)).  `seq` indexes the
@@ -1277,13 +1142,13 @@ template <class T> static int upload(T **d, const std::vector<T> &h)
     return HRW_OK;
 }
 
-// Chunk table for chunks of 64*R cells.
+// Chunk table for chunks of 62*R cells (the output cells of one wavefront).
 static int batch_chunks(hrw_batch_s *b, int R)
 {
     if (b->chunk_R == R) return HRW_OK;
     std::vector<int> cl, cs;
     for (int l = 0; l < b->nlines; ++l)
-        for (int st = 0; st < b->h_n[l]; st += 64 * R) {
+        for (int st = 0; st < b->h_n[l]; st += kOutLanes * R) {
             cl.push_back(l);
             cs.push_back(st);
         }
@@ -1386,13 +1251,6 @@ extern "C" int hrw_batch_set_stream(hrw_batch_t b, hrw_stream_t stream)
     return HRW_OK;
 }
 
-template <int FLUX, bool AMAX, int R>
-static void lbatch(const BatchArgs &a, hipStream_t s)
-{
-    const unsigned grid = (unsigned)((a.nchunks * 64 + kBlock - 1) / kBlock);
-    batch_stage_kernel<FLUX, AMAX, R><<<grid, kBlock, 0, s>>>(a);
-}
-
 static int batch_stage_sets(hrw_batch_s *b, const unsigned long long *ain,
                             unsigned long long *aout, unsigned long long *aclear, int stage,
                             const double *us, const double *u0, double *uo);
@@ -1432,15 +1290,7 @@ static int batch_stage_sets(hrw_batch_s *b, const unsigned long long *ain,
         a.alpha_clear = aclear;
     }
     const bool lf = b->numflux == HRW_FLUX_LAX_FRIEDRICHS;
-    if (b->chunk_R == 4) {
-        if (lf && amax) lbatch<LF, true, 4>(a, b->stream);
-        else if (lf) lbatch<LF, false, 4>(a, b->stream);
-        else lbatch<GODUNOV, false, 4>(a, b->stream);
-    } else {
-        if (lf && amax) lbatch<LF, true, 8>(a, b->stream);
-        else if (lf) lbatch<LF, false, 8>(a, b->stream);
-        else lbatch<GODUNOV, false, 8>(a, b->stream);
-    }
+    launch_batch_stage(lf ? LF : GODUNOV, lf && amax, b->chunk_R, a, b->stream);
     LAUNCH_CHECK();
     return HRW_OK;
 }
@@ -1552,7 +1402,7 @@ extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
     HIP_TRY(hipEventCreate(&e1));
     int best = b->chunk_R;
     float best_ms = 1e30f;
-    for (int R : {4, 8}) {
+    for (int R : {4, 6, 8}) {
         int rc = batch_chunks(b, R);
         if (rc == HRW_OK) {
             (void)hipMemcpyAsync(scratch, u, b->total * sizeof(double), hipMemcpyDeviceToDevice, b->stream);
@@ -1586,7 +1436,8 @@ extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
 extern "C" int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane)
 {
     if (!b) return fail(HRW_ERR_ARG, "null batch");
-    if (cells_per_lane != 4 && cells_per_lane != 8) return fail(HRW_ERR_ARG, "cells_per_lane must be 4 or 8");
+    if (cells_per_lane != 4 && cells_per_lane != 6 && cells_per_lane != 8)
+        return fail(HRW_ERR_ARG, "cells_per_lane must be 4, 6 or 8");
     b->R = cells_per_lane;
     return batch_chunks(b, cells_per_lane);
 }
@@ -1725,6 +1576,31 @@ static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
     return HRW_OK;
 }
 
+// Checkpoint interval for ckpt_every = 0: the largest m whose states (nsteps/m + 1 checkpoints,
+// order*m - 1 stage inputs of one segment, two cotangents) fit in half of the free device
+// memory.  m = nsteps keeps every stage input of the run, so the backward recomputes nothing.
+static int auto_ckpt(int64_t nsteps, int order, int64_t flat)
+{
+    if (nsteps <= 0) return 1;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const double budget = 0.5 * (double)free_b;
+    auto bytes = [&](int64_t m) {
+        return 8.0 * (double)flat * (double)((nsteps + m - 1) / m + 1 + order * m - 1 + 2);
+    };
+    // the states are convex in m with the minimum near sqrt(nsteps/order): search above it
+    int64_t lo = std::max<int64_t>(1, (int64_t)std::sqrt((double)nsteps / order));
+    lo = std::min(lo, nsteps);
+    if (bytes(lo) > budget) return (int)std::min<int64_t>(lo, INT32_MAX);
+    int64_t hi = std::min<int64_t>(nsteps, INT32_MAX / 4);
+    while (lo < hi) {   // largest m in [lo, hi] with bytes(m) <= budget
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (bytes(mid) <= budget) lo = mid;
+        else hi = mid - 1;
+    }
+    return (int)lo;
+}
+
 static bool adjoint_problem_ok(const hrw_problem_s *p)
 {
     return p->kind == PROB_BURGERS1D && p->bc == HRW_BC_PERIODIC &&
@@ -1743,8 +1619,9 @@ extern "C" int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t
         return fail(HRW_ERR_ARG, "Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.");
     if (!(dt != 0.0) || !std::isfinite(dt)) return fail(HRW_ERR_ARG, "dt must be finite and nonzero");
     if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
-    if (ckpt_every < 1) return fail(HRW_ERR_ARG, "ckpt_every must be >= 1");
+    if (ckpt_every < 0) return fail(HRW_ERR_ARG, "ckpt_every must be >= 0 (0 = automatic)");
     if (p->neq >= (1LL << 31) - 64LL * 8) return fail(HRW_ERR_ARG, "the adjoint needs ncells < 2^31 - 512");
+    if (ckpt_every == 0) ckpt_every = auto_ckpt(nsteps, order, p->neq);
     auto *a = new (std::nothrow) hrw_adjoint_s;
     if (!a) return fail(HRW_ERR_ALLOC, "out of host memory");
     std::unique_ptr<hrw_adjoint_s> guard(a);
@@ -1775,7 +1652,8 @@ extern "C" int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_
     if (b->numflux != HRW_FLUX_LAX_FRIEDRICHS)
         return fail(HRW_ERR_ARG, "the adjoint covers the Lax-Friedrichs flux");
     if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
-    if (ckpt_every < 1) return fail(HRW_ERR_ARG, "ckpt_every must be >= 1");
+    if (ckpt_every < 0) return fail(HRW_ERR_ARG, "ckpt_every must be >= 0 (0 = automatic)");
+    if (ckpt_every == 0) ckpt_every = auto_ckpt(nsteps, 3, b->total);
     auto *a = new (std::nothrow) hrw_adjoint_s;
     if (!a) return fail(HRW_ERR_ALLOC, "out of host memory");
     std::unique_ptr<hrw_adjoint_s> guard(a);
@@ -1899,13 +1777,6 @@ extern "C" int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN
     return HRW_OK;
 }
 
-template <int R, bool AMAX>
-static void ladj(const AdjArgs &g, hipStream_t s)
-{
-    const unsigned grid = (unsigned)((g.nchunks * 64 + kBlock - 1) / kBlock);
-    adjoint_stage_kernel<R, AMAX><<<grid, kBlock, 0, s>>>(g);
-}
-
 // out = ca*lam + cc*h*J(X[q])^T lam + e1*x1 + e2*x2 over every line.
 static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *lam, double *out,
                          double ca, double cc, const double *x1, double e1, const double *x2,
@@ -1964,13 +1835,7 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
         a->set_owed(out, mine);   // out's earlier debt (if out aliases x1/x2) is settled here
         ++a->launches;
     }
-    if (a->R == 4) {
-        if (amax) ladj<4, true>(g, a->strm());
-        else ladj<4, false>(g, a->strm());
-    } else {
-        if (amax) ladj<2, true>(g, a->strm());
-        else ladj<2, false>(g, a->strm());
-    }
+    launch_adjoint_stage(a->R, amax, g, a->strm());
     LAUNCH_CHECK();
     return HRW_OK;
 }

@@ -21,7 +21,10 @@
 namespace hrw {
 
 constexpr int kBlock = 256;
-constexpr int kNSlot = 64;       // partial maxima per slot set
+// Partial maxima per slot set: every block atomicMax-es into one of them, every wave of the
+// next launch reads them all (16 cache lines), so fewer means less L2 traffic per wave and
+// more atomics per slot.
+constexpr int kNSlot = 16;
 constexpr int kSlotStride = 16;  // 16 x 8 B = one 128-byte line per partial
 
 __device__ __forceinline__ long long cell_index(long long i, long long n, bool periodic)
@@ -32,6 +35,33 @@ __device__ __forceinline__ long long cell_index(long long i, long long n, bool p
         return r < 0 ? r + n : r;
     }
     return i < 0 ? 0 : n - 1;   // constant extrapolation, src/hrweno_weno.f90:171-173
+}
+
+// Maximum over the wavefront of an unsigned 64-bit value (the bit patterns of non-negative
+// doubles order like the doubles) by DPP, without LDS round trips: after the row shifts by 1,
+// 2, 4, 8 (a lane with no source keeps its own value) every lane holds the maximum of its row
+// up to itself, so lane 15 of each row holds the row's; row_bcast:15 carries rows 0 and 2 into
+// rows 1 and 3, row_bcast:31 carries lane 31 into rows 2 and 3, leaving the wavefront's
+// maximum in lane 63, which is broadcast.  Every lane of the wavefront must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), CTRL, ROW_MASK, 0xF, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long wave_umax(unsigned long long v)
+{
+    auto mx = [](unsigned long long x, unsigned long long y) { return x > y ? x : y; };
+    v = mx(v, dpp_u64<0x111, 0xF>(v));   // row_shr:1
+    v = mx(v, dpp_u64<0x112, 0xF>(v));   // row_shr:2
+    v = mx(v, dpp_u64<0x114, 0xF>(v));   // row_shr:4
+    v = mx(v, dpp_u64<0x118, 0xF>(v));   // row_shr:8
+    v = mx(v, dpp_u64<0x142, 0xA>(v));   // row_bcast:15
+    v = mx(v, dpp_u64<0x143, 0xC>(v));   // row_bcast:31
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -46,7 +76,7 @@ __device__ __forceinline__ double wave_max(double v)
 __device__ __forceinline__ double read_alpha(const unsigned long long *set, double *s_tmp)
 {
     if (threadIdx.x < 64) {
-        double a = __longlong_as_double((long long)set[threadIdx.x * kSlotStride]);
+        double a = __longlong_as_double((long long)set[(threadIdx.x % kNSlot) * kSlotStride]);
         a = wave_max(a);
         if (threadIdx.x == 0) s_tmp[0] = a;
     }
@@ -75,13 +105,13 @@ __device__ __forceinline__ void clear_set(unsigned long long *set)
 }
 
 // --------------------------------------------------------------------------------------
-__global__ void clear_slots_kernel(unsigned long long *slots, int nsets)
+static __global__ void clear_slots_kernel(unsigned long long *slots, int nsets)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nsets * kNSlot) slots[t * kSlotStride] = 0ull;
 }
 
-__global__ __launch_bounds__(kBlock) void maxabs_kernel(const double *__restrict__ u, long long n,
+static __global__ __launch_bounds__(kBlock) void maxabs_kernel(const double *__restrict__ u, long long n,
                                                         unsigned long long *set)
 {
     __shared__ double s_red[kBlock / 64];
@@ -237,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(int stage, long long n,
 
 // Numerical fluxes over arrays of interfaces (src/hrweno_fluxes.f90:22-76) for the built-in
 // flux functions: fn 0 -> v**2/2, fn 1 -> (ca + cb*x)*v.
-__global__ __launch_bounds__(kBlock) void flux_kernel(int which, int fn, double ca, double cb,
+static __global__ __launch_bounds__(kBlock) void flux_kernel(int which, int fn, double ca, double cb,
                                                       long long n, const double *vm,
                                                       const double *vp, const double *x,
                                                       double t, double alpha, double *h)

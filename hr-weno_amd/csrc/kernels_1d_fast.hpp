@@ -161,58 +161,84 @@ __host__ __device__ inline StageCoef stage_coef(int stage, double h)
     }
 }
 
-// One wavefront's chunk [W, W + 64R) of a line: lane l owns the R cells from c0 = W + l*R.
-// Halo: H = K cells on each side (K-1 for the reconstruction window, one more for the
-// neighbouring reconstruction that the chunk-edge interface needs).  Interior chunks (range and
-// halo inside the line) take vector loads/stores and no boundary checks.  NPW = number of
-// pointwise operands of the stage (0, 1 or 2).  Returns the lane's max|output| (the next
-// stage's alpha contribution).  Requires n < 2^31 and 16-byte aligned arrays (checked on the
-// host).  Must be called by all 64 lanes of the wavefront.
-// What an interior chunk reads from memory, held in registers so that it can be loaded one
-// chunk ahead (the persistent kernel below): the lane's R cells of us and the pointwise
-// operands.  (The K-cell halo that only the two wave-edge lanes load is read when used.)
+// One wavefront's chunk of a line: OUTPUT cells [W, W + 62R).  Lane l reconstructs the R cells
+// from c0 = W + (l-1)*R, each once (window halo K-1 cells each side, from the neighbouring
+// lanes by lane shuffles; lanes 0 and 63 load their outer halo).  A lane's first and last
+// cells need a face value from a neighbouring lane (vr of cell c0-1, vl of cell c0+R), which
+// that lane has only at the end of its own march: they are finished after the march, from
+// the values kept for them.  Lanes 0 and 63 are halo lanes: their cells only supply the outer
+// faces of lanes 1 and 62 and they write nothing.  Interior chunks (range and halo inside the
+// line) take vector loads/stores and no boundary checks.  NPW = number of pointwise operands
+// of the stage (0, 1 or 2).  Returns the lane's max|output| (the next stage's alpha
+// contribution).  Requires n < 2^31 and 16-byte aligned arrays (checked on the host).  Must be
+// called by all 64 lanes of the wavefront.
+constexpr int kOutLanes = 62;
+
+// What an interior chunk reads from memory, held in registers so that it can be loaded ahead
+// of its use: the lane's R cells of us, the pointwise operands and, on the two wave-edge
+// lanes, the K-1 halo cells beyond the chunk.  All of it is loaded together, so that a chunk
+// computed from loaded data issues no load after the loads of the next one (a later load
+// would make its wait cover them too).
 template <int K, int R, int NPW>
 struct ChunkData {
     double run[R];
     double p1[NPW >= 1 ? R : 1], p2[NPW >= 2 ? R : 1];
+    double halo[K > 1 ? K - 1 : 1];
 };
 
-__device__ __forceinline__ bool chunk_interior(long long W, int R, int H, long long n)
+__device__ __forceinline__ bool chunk_interior(long long W, int R, int K, long long n)
 {
-    return W >= H && W + 64LL * R + H <= n;
+    return W - R - (K - 1) >= 0 && W + 63LL * R + (K - 1) <= n;
 }
 
 template <int K, int R, int NPW>
 __device__ __forceinline__ void load_chunk(const Stage1DArgs &a, int W, ChunkData<K, R, NPW> &d)
 {
-    const int c0 = W + (int)(threadIdx.x & 63) * R;
+    const int lane = threadIdx.x & 63;
+    const int c0 = W + (lane - 1) * R;
     load_run<R>(a.us + c0, d.run);
     const double *s1 = a.stage == MS3 ? a.uo4 : a.u0;
     if constexpr (NPW >= 1) load_run<R>(s1 + c0, d.p1);
     if constexpr (NPW >= 2) load_run<R>(a.Lo4 + c0, d.p2);
+    // lanes 0..31 read the K-1 cells before lane 0's, lanes 32..63 the K-1 after lane 63's:
+    // two addresses per wave, no divergent branch (lanes 0 and 63 use them)
+    if constexpr (K > 1) {
+        const double *h = a.us + (lane < 32 ? W - R - (K - 1) : W + 63 * R);
+#pragma unroll
+        for (int i = 0; i < K - 1; ++i) d.halo[i] = h[i];
+    }
 }
 
-template <int K, int FLUX, bool PERIODIC, int R, int NPW>
-__device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha,
-                                              const ChunkData<K, R, NPW> *pre = nullptr)
+// The chunks [lo, hi] of a line of n cells are interior (hi < lo: none).
+__device__ __forceinline__ void interior_range(long long n, int R, int K, long long &lo, long long &hi)
 {
-    constexpr int H = K;
+    const long long cw = (long long)kOutLanes * R;
+    const long long need_lo = R + (K - 1), need_hi = 63LL * R + (K - 1);
+    lo = (need_lo + cw - 1) / cw;
+    hi = n >= need_hi ? (n - need_hi) / cw : -1;
+}
+
+// INTERIOR: the chunk is interior and d holds its loads (issued by the caller, so that the
+// stage itself issues no load); otherwise d is not read and the cells are loaded here with
+// the boundary handling.
+template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR>
+__device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha,
+                                              const ChunkData<K, R, NPW> &d)
+{
+    static_assert(R >= 4 && R % 2 == 0, "pairs of cells per lane");
+    constexpr int H = K - 1;
     constexpr int NV = R + 2 * H;
     const int lane = threadIdx.x & 63;
     const int n = (int)a.n;
-    const int c0 = W + lane * R;
+    const int c0 = W + (lane - 1) * R;
+    const bool owned = lane >= 1 && lane <= kOutLanes;
     const StageCoef sc = stage_coef(a.stage, a.h);
     const bool write_u = a.stage >= 0;
 
-    // ---- the lane's cells and halo (v[i] <-> cell c0 - H + i) and pointwise operands
+    // ---- the lane's cells and window halo (v[i] <-> cell c0 - H + i) and pointwise operands
     double v[NV];
     [[maybe_unused]] double p1[NPW >= 1 ? R : 1], p2[NPW >= 2 ? R : 1];
-    const double *s1 = a.stage == MS3 ? a.uo4 : a.u0;
-    const bool interior = chunk_interior(W, R, H, n);
-    if (interior) {
-        ChunkData<K, R, NPW> d;
-        if (pre) d = *pre;
-        else load_chunk<K, R, NPW>(a, W, d);
+    if constexpr (INTERIOR) {
 #pragma unroll
         for (int i = 0; i < R; ++i) v[H + i] = d.run[i];
         if constexpr (NPW >= 1) {
@@ -228,31 +254,76 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
             v[i] = __shfl_up(v[R + i], 1, 64);            // lane-1's last H cells
             v[H + R + i] = __shfl_down(v[H + i], 1, 64);  // lane+1's first H cells
         }
-        if (lane == 0) {
+        if constexpr (H > 0) {
+            if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < H; ++i) v[i] = a.us[c0 - H + i];
-        }
-        if (lane == 63) {
+                for (int i = 0; i < H; ++i) v[i] = d.halo[i];
+            }
+            if (lane == 63) {
 #pragma unroll
-            for (int i = 0; i < H; ++i) v[H + R + i] = a.us[c0 + R + i];
+                for (int i = 0; i < H; ++i) v[H + R + i] = d.halo[i];
+            }
         }
     } else {
+        const double *s1 = a.stage == MS3 ? a.uo4 : a.u0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) v[i] = a.us[cell_index(c0 - H + i, n, PERIODIC)];
+        for (int i = 0; i < NV; ++i) v[i] = a.us[cell_index((long long)c0 - H + i, n, PERIODIC)];
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            const int g = c0 + i < n ? c0 + i : n - 1;
+            const int g = c0 + i < 0 ? 0 : (c0 + i < n ? c0 + i : n - 1);
             if constexpr (NPW >= 1) p1[i] = s1[g];
             if constexpr (NPW >= 2) p2[i] = a.Lo4[g];
         }
     }
+    // a cell is real (inside the line) -- every cell of an interior chunk is
+    auto real = [&](int g) { return INTERIOR || (g >= 0 && g < n); };
 
-    // ---- march over the cells: reconstruction of cell c0-1+s for s = 0..R+1
+    // ---- width of cell g (recomputed from the edges like the reference, or the given array)
+    auto edge = [&](int g) { return linear_edge32(a.xmin, a.rx, g < 0 ? 0 : (g < n ? g : n)); };
+    auto width_of = [&](int g, double e_lo, double e_hi) {
+        return a.width ? a.width[g < 0 ? 0 : (g < n ? g : n - 1)] : e_hi - e_lo;
+    };
+
+    // ---- stage combination and store of cell t (global g); un/L of a pair's first cell wait
     double m_out = 0.0;
-    double fl = 0.0, vr_prev = 0.0;
-    double e_prev = linear_edge32(a.xmin, a.rx, c0 < n ? c0 : n);
+    auto finish = [&](int t, double fleft, double fright, double w, double &un, double &L) {
+        const int g = c0 + t;
+        if constexpr (!PERIODIC) {
+            if (!INTERIOR) {   // example1:103-104
+                if (g == 0) fleft = fright;
+                if (g == n - 1) fright = fleft;
+            }
+        }
+        L = (fleft - fright) * rcp_nr(w);
+        const double us = v[H + t];
+        double tt = sc.A * us;
+        if constexpr (NPW >= 1) tt = fma_(sc.B, p1[t], tt);
+        tt = fma_(sc.C, L, tt);
+        if constexpr (NPW >= 2) tt = fma_(sc.D, p2[t], tt);
+        un = sc.div3 ? div3_fast(tt) : sc.post * tt;
+        if (real(g)) m_out = fmax(m_out, fabs(un));
+    };
+    auto store_pair = [&](int t, double un0, double un1, double L0, double L1) {   // cells t, t+1
+        if (!owned) return;
+        const int g = c0 + t;
+        if constexpr (INTERIOR) {
+            if (write_u) *reinterpret_cast<double2 *>(a.uo + g) = make_double2(un0, un1);
+            if (a.Lout) *reinterpret_cast<double2 *>(a.Lout + g) = make_double2(L0, L1);
+        } else {
+            if (g >= 0 && g < n) {
+                if (write_u) a.uo[g] = un0;
+                if (a.Lout) a.Lout[g] = L0;
+            }
+            if (g + 1 >= 0 && g + 1 < n) {
+                if (write_u) a.uo[g + 1] = un1;
+                if (a.Lout) a.Lout[g + 1] = L1;
+            }
+        }
+    };
+
+    // ---- march over the lane's cells t = 0..R-1: reconstruction of cell c0+t; the interface
+    // between cells t-1 and t; cell t-1 finished for t >= 2
     const double eps12 = 12.0 * a.eps;
-    double un_pair[2], L_pair[2];
     [[maybe_unused]] Tri3 t2, t1;
     [[maybe_unused]] Pair2 pp1;
     if constexpr (K == 3) {
@@ -261,162 +332,216 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     } else if constexpr (K == 2) {
         pp1 = pair2(v[0], v[1]);
     }
+    double vl0 = 0.0, vr_prev = 0.0, f_first = 0.0, fl = 0.0;
+    const double e_c0 = edge(c0), e_c1 = edge(c0 + 1);
+    double e_prev = e_c1, e_last = 0.0;
+    double un_pair = 0.0, L_pair = 0.0, un1 = 0.0, L1 = 0.0, unR2 = 0.0, LR2 = 0.0;
 #pragma unroll
-    for (int s = 0; s <= R + 1; ++s) {
+    for (int t = 0; t < R; ++t) {
         double vl, vr;
         if constexpr (K == 3) {
-            const Tri3 t0 = tri3(v[s + 2], v[s + 3], v[s + 4]);   // T(c0-1+s)
+            const Tri3 t0 = tri3(v[t + 2], v[t + 3], v[t + 4]);   // T(c0+t)
             recon3(eps12, t2, t1, t0, vl, vr);
             t2 = t1;
             t1 = t0;
         } else if constexpr (K == 2) {
-            const Pair2 p0 = pair2(v[s + 1], v[s + 2]);
+            const Pair2 p0 = pair2(v[t + 1], v[t + 2]);
             recon2(a.eps, pp1, p0, vl, vr);
             pp1 = p0;
         } else {
-            vl = vr = v[s];
+            vl = vr = v[t];
         }
-        if (s == 0) {
+        if (t == 0) {
+            vl0 = vl;
             vr_prev = vr;
             continue;
         }
-        // interface (c0-1+s) - 1/2 between cells c0-2+s and c0-1+s
-        const double f = burgers_numflux<FLUX, true>(vr_prev, vl, alpha);
+        const double f = burgers_numflux<FLUX, true>(vr_prev, vl, alpha);   // face c0+t-1/2
         vr_prev = vr;
-        if (s == 1) {
+        if (t == 1) {
+            f_first = f;
             fl = f;
             continue;
         }
-        // cell g = c0 + i, i = s - 2, between fluxes fleft and fr
-        const int i = s - 2;
-        const int g = c0 + i;
-        double fr = f, fleft = fl;
+        // cell t-1 (1 <= t-1 <= R-2) between fl and f
+        const int c = t - 1;
+        const double e = edge(c0 + c + 1);
+        const double w = width_of(c0 + c, e_prev, e);
+        e_prev = e;
+        double un, L;
+        finish(c, fl, f, w, un, L);
         fl = f;
-        double w;
-        if (a.width) {
-            w = a.width[g < n ? g : n - 1];
+        if (c == 1) {
+            un1 = un;
+            L1 = L;
+        } else if (c == R - 2) {
+            unR2 = un;
+            LR2 = L;
+            e_last = e;
+        } else if (c % 2 == 0) {
+            un_pair = un;
+            L_pair = L;
         } else {
-            const double e = linear_edge32(a.xmin, a.rx, g + 1);
-            w = e - e_prev;
-            e_prev = e;
-        }
-        if constexpr (!PERIODIC) {
-            if (!interior) {   // example1:103-104
-                if (g == 0) fleft = fr;
-                if (g == n - 1) fr = fleft;
-            }
-        }
-        const double L = (fleft - fr) * rcp_nr(w);
-        const double us = v[H + i];
-        double t = sc.A * us;
-        if constexpr (NPW >= 1) t = fma_(sc.B, p1[i], t);
-        t = fma_(sc.C, L, t);
-        if constexpr (NPW >= 2) t = fma_(sc.D, p2[i], t);
-        const double un = sc.div3 ? div3_fast(t) : sc.post * t;
-        // write back in pairs (16-byte stores) inside the line, one by one at its ends
-        un_pair[i & 1] = un;
-        L_pair[i & 1] = L;
-        if (interior) {
-            m_out = fmax(m_out, fabs(un));
-            if (i & 1) {
-                if (write_u) *reinterpret_cast<double2 *>(a.uo + g - 1) = make_double2(un_pair[0], un_pair[1]);
-                if (a.Lout) *reinterpret_cast<double2 *>(a.Lout + g - 1) = make_double2(L_pair[0], L_pair[1]);
-            }
-        } else if (g < n) {
-            m_out = fmax(m_out, fabs(un));
-            if (write_u) a.uo[g] = un;
-            if (a.Lout) a.Lout[g] = L;
+            store_pair(c - 1, un_pair, un, L_pair, L);
         }
     }
-    return m_out;
+    // ---- the two faces shared with the neighbouring lanes, then cells 0 and R-1
+    const double vr_left = __shfl_up(vr_prev, 1, 64);    // vr of cell c0-1 (lane-1's last)
+    const double vl_right = __shfl_down(vl0, 1, 64);     // vl of cell c0+R (lane+1's first)
+    {
+        const double f0 = burgers_numflux<FLUX, true>(vr_left, vl0, alpha);
+        double un, L;
+        finish(0, f0, f_first, width_of(c0, e_c0, e_c1), un, L);
+        store_pair(0, un, un1, L, L1);
+    }
+    {
+        const double fR = burgers_numflux<FLUX, true>(vr_prev, vl_right, alpha);
+        double un, L;
+        finish(R - 1, fl, fR, width_of(c0 + R - 1, e_last, edge(c0 + R)), un, L);
+        store_pair(R - 2, unR2, un, LR2, L);
+    }
+    return owned ? m_out : 0.0;
 }
 
-template <int K, int FLUX, bool PERIODIC, int R>
-__device__ __forceinline__ double chunk_stage_any(const Stage1DArgs &a, int W, double alpha)
-{
-    if (a.stage == MS3) return chunk_stage<K, FLUX, PERIODIC, R, 2>(a, W, alpha);
-    if (a.stage == RK2_2 || a.stage == RK3_2 || a.stage == RK3_3)
-        return chunk_stage<K, FLUX, PERIODIC, R, 1>(a, W, alpha);
-    return chunk_stage<K, FLUX, PERIODIC, R, 0>(a, W, alpha);
-}
-
-#ifndef HRW_FAST_WAVES_PER_EU
-#define HRW_FAST_WAVES_PER_EU 4
-#endif
-
-// Persistent form: wave w of the grid takes chunks w, w + N, w + 2N, ... (N = waves in the
-// grid) and loads the next interior chunk into a second register set before it computes the
-// current one, so every wave keeps a chunk of loads in flight behind its FP64 work.
 template <int K, int FLUX, bool PERIODIC, int R, int NPW>
-__device__ __forceinline__ double persistent_stage(const Stage1DArgs &a, double alpha)
+__device__ __forceinline__ double chunk_load_stage(const Stage1DArgs &a, int W, double alpha)
 {
-    const long long nw = (long long)gridDim.x * (kBlock / 64);
-    const long long nch = (a.n + 64LL * R - 1) / (64LL * R);
-    long long c = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    ChunkData<K, R, NPW> cur, nxt;
-    double m = 0.0;
-    if (c < nch && chunk_interior(c * 64 * R, R, K, a.n)) load_chunk<K, R, NPW>(a, (int)(c * 64 * R), cur);
-    for (; c < nch; c += nw) {
-        const long long c1 = c + nw;
-        if (c1 < nch && chunk_interior(c1 * 64 * R, R, K, a.n)) load_chunk<K, R, NPW>(a, (int)(c1 * 64 * R), nxt);
-        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW>(a, (int)(c * 64 * R), alpha, &cur));
-        cur = nxt;
+    ChunkData<K, R, NPW> d;
+    if (chunk_interior(W, R, K, a.n)) {
+        load_chunk<K, R, NPW>(a, W, d);
+        return chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, W, alpha, d);
     }
-    return m;
+    return chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, W, alpha, d);
 }
 
-template <int K, int FLUX, bool PERIODIC, int R>
-__device__ __forceinline__ double persistent_stage_any(const Stage1DArgs &a, double alpha)
+// Pointwise operands of a stage: u0 for the second and third RK stages, u_{n-4} and
+// L(u_{n-4}) for the multistep stage, none otherwise.
+__host__ __device__ constexpr int stage_npw(int stage)
 {
-    if (a.stage == MS3) return persistent_stage<K, FLUX, PERIODIC, R, 2>(a, alpha);
-    if (a.stage == RK2_2 || a.stage == RK3_2 || a.stage == RK3_3)
-        return persistent_stage<K, FLUX, PERIODIC, R, 1>(a, alpha);
-    return persistent_stage<K, FLUX, PERIODIC, R, 0>(a, alpha);
+    return stage == MS3 ? 2 : ((stage == RK2_2 || stage == RK3_2 || stage == RK3_3) ? 1 : 0);
 }
 
-#ifndef HRW_PF_WAVES_PER_EU
-#define HRW_PF_WAVES_PER_EU 3
-#endif
+// Waves per SIMD each stage kernel is compiled for: the most whose register budget (512 /
+// waves) holds the march without spilling (R = 8 needs 136..168 VGPRs, R = 6 112..144, R = 4
+// 102..120 for 0..2 pointwise operands).
+__host__ __device__ constexpr int fast_waves(int R, int NPW)
+{
+    return R >= 8 ? 3 : (R == 6 ? (NPW <= 1 ? 4 : 3) : 4);
+}
 
-template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_PF_WAVES_PER_EU)))
+// The start of every stage wave: the alpha partial maxima of the previous launch are loaded
+// first and the wave's first chunk right after them, so that the reduction of the partials
+// waits for those loads only (vector memory counters complete in issue order) and overlaps
+// the chunk's loads; then this launch's alpha is recorded and the slot set two launches ahead
+// cleared.
+template <bool ALPHAMAX>
+__device__ __forceinline__ unsigned long long alpha_partial(const Stage1DArgs &a)
+{
+    if constexpr (ALPHAMAX) return a.alpha_in[((threadIdx.x & 63) % kNSlot) * kSlotStride];
+    else return 0ull;
+}
+template <bool ALPHAMAX>
+__device__ __forceinline__ double alpha_reduce(const Stage1DArgs &a, unsigned long long part)
+{
+    double alpha = a.alpha;
+    if constexpr (ALPHAMAX) {
+        alpha = __longlong_as_double((long long)wave_umax(part));
+        if (a.alpha_record && blockIdx.x == 0 && threadIdx.x == 0) *a.alpha_record = alpha;
+    }
+    clear_set(a.alpha_clear);
+    return alpha;
+}
+
+// Persistent form: wave w of the grid takes the interior chunks lo + w, lo + w + N, ... (N =
+// waves in the grid) and loads the next one into a second register set before it computes the
+// current one, so every wave keeps a chunk of loads in flight behind its FP64 work.  The loads
+// are unconditional (the last iteration reloads its own chunk, an L2 hit): a load on only some
+// paths would make the compiler's wait for the current chunk cover the prefetch as well.  The
+// few boundary chunks follow, one per wave.
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+__device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_red)
+{
+    const unsigned long long part = alpha_partial<ALPHAMAX>(a);
+    const long long nw = (long long)gridDim.x * (kBlock / 64);
+    const long long cw = (long long)kOutLanes * R;
+    const long long nch = (a.n + cw - 1) / cw;
+    long long lo, hi;
+    interior_range(a.n, R, K, lo, hi);
+    const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
+    double m = 0.0, alpha;
+    ChunkData<K, R, NPW> cur, nxt;
+    if (lo <= hi) {
+        long long c = lo + w;
+        load_chunk<K, R, NPW>(a, (int)((c <= hi ? c : hi) * cw), cur);
+        alpha = alpha_reduce<ALPHAMAX>(a, part);
+        for (; c <= hi; c += nw) {
+            const long long c1 = c + nw <= hi ? c + nw : c;
+            load_chunk<K, R, NPW>(a, (int)(c1 * cw), nxt);
+            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, (int)(c * cw), alpha, cur));
+            cur = nxt;
+        }
+    } else {
+        alpha = alpha_reduce<ALPHAMAX>(a, part);
+    }
+    // boundary chunks [0, lo) and (hh, nch), hh = max(hi, lo - 1)
+    const long long hh = hi >= lo ? hi : lo - 1;
+    const long long nb = lo + (nch - 1 - hh);
+    for (long long b = w; b < nb; b += nw) {
+        const long long c = b < lo ? b : hh + 1 + (b - lo);
+        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, (int)(c * cw), alpha, cur));
+    }
+    if constexpr (ALPHAMAX) publish_max(m, a.alpha_out, s_red);
+}
+
+// (The second register set of the prefetch costs one wave per SIMD against the plain form.)
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW) > 2 ? fast_waves(R, NPW) - 1 : 2)))
 void stage1d_pf_kernel(Stage1DArgs a)
 {
     __shared__ double s_red[kBlock / 64 + 1];
-    const int lane = threadIdx.x & 63;
-    double alpha = a.alpha;
-    if constexpr (ALPHAMAX) {
-        alpha = wave_max(__longlong_as_double((long long)a.alpha_in[lane * kSlotStride]));
-        if (a.alpha_record && blockIdx.x == 0 && threadIdx.x == 0) *a.alpha_record = alpha;
+    persistent_wave<K, FLUX, PERIODIC, ALPHAMAX, R, NPW>(a, s_red);
+}
+
+// One chunk per wave: wave w of block b owns the chunk starting at W = (4b + w)*62R.
+// When the line has interior chunks every wave loads one unconditionally (a boundary wave
+// the nearest interior chunk, unused), so that no join of paths with and without loads comes
+// before the alpha reduction, whose wait would then cover the chunk's loads.
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+__device__ __forceinline__ void chunk_wave(const Stage1DArgs &a, double *s_red)
+{
+    const long long cw = (long long)kOutLanes * R;
+    const long long c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
+    const int W = (int)(c * cw);
+    long long lo, hi;
+    interior_range(a.n, R, K, lo, hi);
+    const unsigned long long part = alpha_partial<ALPHAMAX>(a);
+    ChunkData<K, R, NPW> d;
+    double m_out;
+    if (lo <= hi) {
+        load_chunk<K, R, NPW>(a, (int)((c < lo ? lo : (c > hi ? hi : c)) * cw), d);
+        const double alpha = alpha_reduce<ALPHAMAX>(a, part);
+        if (c >= lo && c <= hi) m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, W, alpha, d);
+        else m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, W, alpha, d);
+    } else {
+        const double alpha = alpha_reduce<ALPHAMAX>(a, part);
+        m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, W, alpha, d);
     }
-    clear_set(a.alpha_clear);
-    const double m_out = persistent_stage_any<K, FLUX, PERIODIC, R>(a, alpha);
     if constexpr (ALPHAMAX) publish_max(m_out, a.alpha_out, s_red);
 }
 
-// One line: wave w of block b owns the chunk starting at W = (4b + w)*64R.
-template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_FAST_WAVES_PER_EU)))
+// NPW = stage_npw(a.stage), chosen by the host.
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW))))
 void stage1d_fast_kernel(Stage1DArgs a)
 {
     __shared__ double s_red[kBlock / 64 + 1];
-    const int lane = threadIdx.x & 63;
-    const int W = (int)(((long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64LL * R));
-    double alpha = a.alpha;
-    if constexpr (ALPHAMAX) {
-        // every wave reduces the 64 partial maxima itself (no barrier)
-        alpha = wave_max(__longlong_as_double((long long)a.alpha_in[lane * kSlotStride]));
-        if (a.alpha_record && blockIdx.x == 0 && threadIdx.x == 0) *a.alpha_record = alpha;
-    }
-    clear_set(a.alpha_clear);
-    const double m_out = chunk_stage_any<K, FLUX, PERIODIC, R>(a, W, alpha);
-    if constexpr (ALPHAMAX) publish_max(m_out, a.alpha_out, s_red);
+    chunk_wave<K, FLUX, PERIODIC, ALPHAMAX, R, NPW>(a, s_red);
 }
 
 // --------------------------------------------------------------------------------------
 // Many independent periodic lines in one launch.  Line l: cells [off[l], off[l] + n[l]) of
 // the flat state, WENO order k[l], step h[l], linear grid xmin[l] + rx[l]*i.  Chunk c of the
-// table covers cells [start[c], start[c] + 64R) of line line[c]; wavefront c of the grid takes
+// table covers cells [start[c], start[c] + 62R) of line line[c]; wavefront c of the grid takes
 // chunk c.  Lax-Friedrichs alpha = max|v| per line: slot set s holds one partial per line.
 struct BatchArgs {
     const double *us, *u0;
@@ -433,11 +558,12 @@ struct BatchArgs {
     unsigned long long *alpha_clear;      // nlines partials, cleared
 };
 
-template <int FLUX, bool ALPHAMAX, int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HRW_FAST_WAVES_PER_EU)))
+template <int FLUX, bool ALPHAMAX, int R, int NPW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW))))
 void batch_stage_kernel(BatchArgs b)
 {
-    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    // wave-uniform, so the chunk table and the line's parameters come through scalar loads
+    const long long wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
     const int lane = threadIdx.x & 63;
     if (ALPHAMAX && b.alpha_clear) {
         for (long long l = (long long)blockIdx.x * kBlock + threadIdx.x; l < b.nlines;
@@ -462,9 +588,9 @@ void batch_stage_kernel(BatchArgs b)
     const int W = b.chunk_start[wave];
     double m;
     switch (b.k[line]) {
-    case 1: m = chunk_stage_any<1, FLUX, true, R>(a, W, alpha); break;
-    case 2: m = chunk_stage_any<2, FLUX, true, R>(a, W, alpha); break;
-    default: m = chunk_stage_any<3, FLUX, true, R>(a, W, alpha); break;
+    case 1: m = chunk_load_stage<1, FLUX, true, R, NPW>(a, W, alpha); break;
+    case 2: m = chunk_load_stage<2, FLUX, true, R, NPW>(a, W, alpha); break;
+    default: m = chunk_load_stage<3, FLUX, true, R, NPW>(a, W, alpha); break;
     }
     if constexpr (ALPHAMAX) {
         m = wave_max(m);
@@ -473,7 +599,7 @@ void batch_stage_kernel(BatchArgs b)
 }
 
 // max|u| of every line into slot set `set` (one partial per line).
-__global__ __launch_bounds__(kBlock) void batch_maxabs_kernel(const double *__restrict__ u,
+static __global__ __launch_bounds__(kBlock) void batch_maxabs_kernel(const double *__restrict__ u,
                                                               const long long *off, const int *n,
                                                               const int *chunk_line,
                                                               const int *chunk_start,
@@ -485,14 +611,14 @@ __global__ __launch_bounds__(kBlock) void batch_maxabs_kernel(const double *__re
     if (wave >= nchunks) return;
     const int line = chunk_line[wave];
     const long long base = off[line] + chunk_start[wave];
-    const int len = min(64 * R, n[line] - chunk_start[wave]);
+    const int len = min(kOutLanes * R, n[line] - chunk_start[wave]);
     double m = 0.0;
     for (int i = lane; i < len; i += 64) m = fmax(m, fabs(u[base + i]));
     m = wave_max(m);
     if (lane == 0) atomicMax(set + line, (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ void fill_u64_kernel(unsigned long long *p, long long count)
+static __global__ void fill_u64_kernel(unsigned long long *p, long long count)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (long long)gridDim.x * blockDim.x)

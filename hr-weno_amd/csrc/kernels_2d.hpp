@@ -108,6 +108,16 @@ __device__ __forceinline__ double dpp_shift(double old, double src)
 __device__ __forceinline__ double from_left(double old, double v) { return dpp_shift<0x138>(old, v); }
 __device__ __forceinline__ double from_right(double old, double v) { return dpp_shift<0x130>(old, v); }
 
+// A copy of x in a register of its own.  A loaded value that becomes loop-carried state (the
+// newest value of the column window) would otherwise keep the load's destination register
+// alive, and the next load into that register would first copy it out, waiting for the load.
+__device__ __forceinline__ double fresh(double x)
+{
+    double y;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs.  This maps block b to a logical block so
 // that each XCD works on one contiguous range of logical blocks (the rows a band of
 // wavefronts shares with the bands above and below stay in that XCD's L2).  A bijection on
@@ -137,15 +147,17 @@ __device__ __forceinline__ void x1_window(double c, double halo, double *x)
     }
 }
 
-// Work decomposition: one wavefront owns a strip of 64 columns and TY rows; the loads of row
-// j+1 (the new x2 window value, the x1 halo, the stage operands) are issued before row j is
-// computed.
-template <int K, int BC, bool FAST>
-__global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
+// Work decomposition: one wavefront owns a strip of 64 columns and TY rows.  The loads of row
+// j+1 (the new x2 window value, the x1 halo, the stage's pointwise operands) are issued before
+// row j is computed, by every lane and on every row (rows and columns clamped into the grid):
+// a load on only some paths would make the compiler's wait for row j's values cover them too.
+// NPW = pointwise operands of the stage (0; 1: u0; 2: u_{n-4}, L(u_{n-4})).
+template <int K, int BC, bool FAST, int NPW>
+__device__ __forceinline__ void pbe2d_wave(const Stage2DArgs &a)
 {
     const int lane = threadIdx.x & 63;
     const unsigned blk = xcd_major(blockIdx.x, gridDim.x);
-    const long long wave = (long long)blk * (kBlock / 64) + (threadIdx.x >> 6);
+    const long long wave = __builtin_amdgcn_readfirstlane((int)(blk * (kBlock / 64) + (threadIdx.x >> 6)));
     const int sx = (int)(wave % a.nstrips);
     const long long sy = wave / a.nstrips;
     const int n1 = a.n1, n2 = a.n2;
@@ -156,8 +168,10 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
     auto colc = [n1](int c) { return c < 0 ? 0 : (c >= n1 ? n1 - 1 : c); };
     const int ic = colc(i);                             // clamped column for loads
     const bool own = lane >= 1 && lane <= kStrip && i < n1;
-    const bool has_halo = lane < K - 1 || lane >= 64 - (K - 1);
-    const int hc = colc(lane < 32 ? i - (K - 1) : i + (K - 1));
+    // halo column: lanes 0..K-2 need i-(K-1), lanes 64-(K-1)..63 need i+(K-1); the other lanes
+    // read the nearest halo lane's column (no extra cache lines)
+    const int hl = lane < 32 ? min(lane, K - 2) : max(lane, 64 - (K - 1));
+    const int hc = colc(sx * kStrip - 1 + hl + (lane < 32 ? -(K - 1) : K - 1));
 
     // per-lane constants: x1 width, g1 at the right interface x_{i+1/2}
     const double e_l = linear_edge(a.x1min, a.rx1, ic), e_r = linear_edge(a.x1min, a.rx1, ic + 1);
@@ -166,8 +180,6 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
     auto rowc = [n2](long long j) -> long long { return j < 0 ? 0 : (j >= n2 ? n2 - 1 : j); };
     const double *us = a.us;
     const int stage = a.stage;
-    const bool need_u0 = stage == RK2_2 || stage == RK3_2 || stage == RK3_3;
-    const bool need_ms = stage == MS3;
 
     // column window for the x2 reconstruction of row r: cw[m] = u(i, r-K+1+m)
     ColumnRecon<K, FAST> col;
@@ -189,38 +201,27 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
         }
     }
 
-    // operands of row j, loaded one row ahead
-    double nx_push = us[rowc(Y0 + K) * n1 + ic];
-    double nx_halo = has_halo ? us[Y0 * n1 + hc] : 0.0;
-    double nx_z = 0.0, nx_u4 = 0.0, nx_l4 = 0.0;
-    if (own) {
-        const long long g = Y0 * n1 + i;
-        if (need_u0) nx_z = a.u0[g];
-        if (need_ms) {
-            nx_u4 = a.uo4[g];
-            nx_l4 = a.Lo4[g];
+    // operands of row r (r may be Y1: clamped, loaded and not used)
+    struct RowOps {
+        double push, halo, p1, p2;
+    };
+    auto load_row = [&](long long r, RowOps &o) {
+        const long long rr = rowc(r);
+        o.push = us[rowc(r + K) * n1 + ic];
+        o.halo = us[rr * n1 + hc];
+        if constexpr (NPW == 1) o.p1 = a.u0[rr * n1 + ic];
+        if constexpr (NPW == 2) {
+            o.p1 = a.uo4[rr * n1 + ic];
+            o.p2 = a.Lo4[rr * n1 + ic];
         }
-    }
+    };
 
     const double iw1 = FAST ? rcp_nr(w1) : 0.0;
-    for (int j = (int)Y0; j < Y1; ++j) {
-        const double push = nx_push, halo = nx_halo, z0 = nx_z, u4 = nx_u4, l4 = nx_l4;
-        if (j + 1 < Y1) {
-            nx_push = us[rowc((long long)j + 1 + K) * n1 + ic];
-            if (has_halo) nx_halo = us[(long long)(j + 1) * n1 + hc];
-            if (own) {
-                const long long g = (long long)(j + 1) * n1 + i;
-                if (need_u0) nx_z = a.u0[g];
-                if (need_ms) {
-                    nx_u4 = a.uo4[g];
-                    nx_l4 = a.Lo4[g];
-                }
-            }
-        }
+    auto row = [&](int j, const RowOps &o) {
         const double uc = col.cw[K - 1];                // u(i, j)
 
         // x2: reconstruction of row j+1, flux at j+1/2
-        col.push(push);
+        col.push(fresh(o.push));
         double vl2n, vr2n;
         col.recon_current(a.eps, vl2n, vr2n);
         const double e2l = linear_edge(a.x2min, a.rx2, j), e2r = linear_edge(a.x2min, a.rx2, j + 1);
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
 
         // x1: reconstruction of (i, j), flux at i+1/2 with the right neighbour's vl
         double xw[2 * K - 1];
-        x1_window<K>(uc, halo, xw);
+        x1_window<K>(uc, o.halo, xw);
         double vl1, vr1;
         recon<K, FAST>(xw, a.eps, vl1, vr1);
         const double vl1_right = from_right(vl1, vl1);
@@ -253,8 +254,8 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
             if (stage < 0) {
                 a.Lout[g] = L;
             } else {
-                const double z = need_u0 ? z0 : uc;
-                const double un = stage_update<FAST>(stage, z, uc, L, a.h, u4, l4);
+                const double z = NPW == 1 ? o.p1 : uc;
+                const double un = stage_update<FAST>(stage, z, uc, L, a.h, o.p1, o.p2);
                 if (a.Lout) a.Lout[g] = L;
                 a.uo[g] = un;
             }
@@ -262,7 +263,27 @@ __global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
         f2_dn = f2_up;
         vl2c = vl2n;
         vr2c = vr2n;
+    };
+
+    // two operand sets in turn: a set is reloaded only after its row is done, so no register
+    // copy (which would wait for the load) sits between a load and its use
+    RowOps A{0.0, 0.0, 0.0, 0.0}, B{0.0, 0.0, 0.0, 0.0};
+    load_row(Y0, A);
+    for (int j = (int)Y0; j < Y1; j += 2) {
+        load_row(j + 1, B);
+        row(j, A);
+        if (j + 1 >= Y1) break;
+        load_row(j + 2, A);
+        row(j + 1, B);
     }
+}
+
+template <int K, int BC, bool FAST>
+__global__ __launch_bounds__(kBlock) void pbe2d_kernel(Stage2DArgs a)
+{
+    if (a.stage == MS3) pbe2d_wave<K, BC, FAST, 2>(a);
+    else if (a.stage == RK2_2 || a.stage == RK3_2 || a.stage == RK3_3) pbe2d_wave<K, BC, FAST, 1>(a);
+    else pbe2d_wave<K, BC, FAST, 0>(a);
 }
 
 }  // namespace hrw

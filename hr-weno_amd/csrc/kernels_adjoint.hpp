@@ -268,11 +268,12 @@ __device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pa
 
 // One wavefront's chunk of a line (see the header comment).  Returns the lane's share of
 // alpha-bar over its owned cells; *cand = the smallest owned cell with |us| == alpha.
-template <int K, int R, bool ALPHAMAX>
+template <int K, int R, bool ALPHAMAX, bool INTERIOR>
 __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *s,
                                                 const double *lam, const double *x1,
                                                 const double *x2, double *out, int n, int W,
-                                                double h, double xmin, double rx, double alpha,
+                                                double h, double xmin, double rx,
+                                                unsigned long long alpha_part,
                                                 int line, long long off, int first, int nc,
                                                 unsigned *cand)
 {
@@ -282,32 +283,58 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     const int lane = threadIdx.x & 63;
     const long long c0 = (long long)W + (long long)(lane - 1) * R;
     const bool owned = lane >= 1 && lane <= 62;
-    const bool interior = (long long)W - R - H >= 0 && (long long)W + 63LL * R + H <= n;
+    constexpr bool interior = INTERIOR;
 
+    // ---- every load of the chunk first, so that their latencies overlap: the lane's cells of
+    // s and lam, the window halo and the lam just outside the two wave-edge lanes (both read
+    // branch-free: lanes 0..31 at lane 0's positions, lanes 32..63 at lane 63's), x1 and x2
     double v[NV], mu[R], phi[R + 2], acc[NV];
-    if (interior) {
+    double p1v[R], p2v[R];
+    [[maybe_unused]] double hv_in[H > 0 ? H : 1];
+    double lam_edge;
+    if constexpr (interior) {
         load_run<R>(s + c0, v + H);
+        if constexpr (H > 0) {
+            const double *hp = s + (lane < 32 ? (long long)W - R - H : (long long)W + 63LL * R);
+#pragma unroll
+            for (int i = 0; i < H; ++i) hv_in[i] = hp[i];
+        }
         load_run<R>(lam + c0, mu);
+        // (in range for K >= 2; for K = 1 the value is not used and the index is clamped)
+        lam_edge = lam[lane < 32 ? max((long long)W - R - 1, 0LL) : min((long long)W + 63LL * R, (long long)n - 1)];
+        if (x1) load_run<R>(x1 + c0, p1v);
+        if (x2) load_run<R>(x2 + c0, p2v);
 #pragma unroll
         for (int i = 0; i < H; ++i) {
             v[i] = __shfl_up(v[R + i], 1, 64);
             v[H + R + i] = __shfl_down(v[H + i], 1, 64);
         }
         // the wave-edge lanes have no neighbour to shuffle from
-        if (lane == 0) {
+        if constexpr (H > 0) {
+            if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < H; ++i) v[i] = s[c0 - H + i];
-        }
-        if (lane == 63) {
+                for (int i = 0; i < H; ++i) v[i] = hv_in[i];
+            }
+            if (lane == 63) {
 #pragma unroll
-            for (int i = 0; i < H; ++i) v[H + R + i] = s[c0 + R + i];
+                for (int i = 0; i < H; ++i) v[H + R + i] = hv_in[i];
+            }
         }
     } else {
 #pragma unroll
         for (int i = 0; i < NV; ++i) v[i] = s[cell_index(c0 - H + i, n, true)];
 #pragma unroll
         for (int i = 0; i < R; ++i) mu[i] = lam[cell_index(c0 + i, n, true)];
+        lam_edge = lam[cell_index(lane < 32 ? c0 - 1 : c0 + R, n, true)];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const bool in = owned && c0 + i < n;
+            p1v[i] = (x1 && in) ? x1[c0 + i] : 0.0;
+            p2v[i] = (x2 && in) ? x2[c0 + i] : 0.0;
+        }
     }
+    // alpha: the partial(s) were loaded before these loads, so this waits for them only
+    const double alpha = ALPHAMAX ? __longlong_as_double((long long)wave_umax(alpha_part)) : a.alpha;
     long long cell[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) cell[i] = interior ? c0 + i : cell_index(c0 + i, n, true);
@@ -327,7 +354,7 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
         const bool edge = lane == 0 || lane == 63;
         const long long pos = lane == 0 ? c0 - 1 : c0 + R;
         long long hc[1] = {interior ? pos : cell_index(pos, n, true)};
-        double hv[1] = {edge ? lam[hc[0]] : 0.0};
+        double hv[1] = {lam_edge};
         apply_pending<1>(a.pend[0], line, off, h, first, nc, hc, edge, hv);
         if (edge) {
             const double w = linear_edge(xmin, rx, hc[0] + 1) - linear_edge(xmin, rx, hc[0]);
@@ -387,23 +414,6 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     }
     // out = a*lam + c*h*acc + e1*x1 + e2*x2 on the owned cells (x1, x2 with their own
     // pending corrections)
-    double p1v[R], p2v[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) p1v[i] = p2v[i] = 0.0;
-    if (owned) {
-        if (interior) {
-            if (x1) load_run<R>(x1 + c0, p1v);
-            if (x2) load_run<R>(x2 + c0, p2v);
-        } else {
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (c0 + i < n) {
-                    if (x1) p1v[i] = x1[c0 + i];
-                    if (x2) p2v[i] = x2[c0 + i];
-                }
-            }
-        }
-    }
     if (x1) apply_pending<R>(a.pend[1], line, off, h, first, nc, cell, owned, p1v);
     if (x2) apply_pending<R>(a.pend[2], line, off, h, first, nc, cell, owned, p2v);
     if (owned) {
@@ -434,7 +444,8 @@ template <int R, bool ALPHAMAX>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R == 2 ? 3 : 2)))
 void adjoint_stage_kernel(AdjArgs a)
 {
-    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    // wave-uniform: the chunk table and the line's parameters come through scalar loads
+    const long long wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
     const int lane = threadIdx.x & 63;
     __shared__ double s_part[kBlock / 64];
     if (wave >= a.nchunks || a.chunk_line[wave] < 0) {   // idle chunks only join the block sum
@@ -448,35 +459,34 @@ void adjoint_stage_kernel(AdjArgs a)
     const long long off = a.off[line];
     const int n = a.n[line];
     const double h = a.h[line];
-    double alpha = a.alpha;
+    // the alpha partials (reduced inside the chunk, after its loads are issued)
+    unsigned long long apart = 0ull;
     if constexpr (ALPHAMAX) {
         const unsigned long long *set = a.aset + (long long)line * a.aline_stride;
-        const double p = lane < a.anpart ? __longlong_as_double((long long)set[lane * a.apart_stride]) : 0.0;
-        alpha = wave_max(p);
+        // lanes past the partials repeat the last one (the maximum is unchanged)
+        apart = set[min(lane, a.anpart - 1) * a.apart_stride];
     }
     const double *x1 = a.x1 ? a.x1 + off : nullptr;
     const double *x2 = a.x2 ? a.x2 + off : nullptr;
     const int first = a.line_chunk0[line], nc = a.line_nchunks[line];
     const int W = a.chunk_start[wave];
+    const int K = a.k[line];
     unsigned cand;
     double ab;
-    switch (a.k[line]) {
-    case 1:
-        ab = chunk_adjoint<1, R, ALPHAMAX>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,
-                                            a.xmin[line], a.rx[line], alpha, line, off, first,
-                                            nc, &cand);
-        break;
-    case 2:
-        ab = chunk_adjoint<2, R, ALPHAMAX>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,
-                                            a.xmin[line], a.rx[line], alpha, line, off, first,
-                                            nc, &cand);
-        break;
-    default:
-        ab = chunk_adjoint<3, R, ALPHAMAX>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,
-                                            a.xmin[line], a.rx[line], alpha, line, off, first,
-                                            nc, &cand);
-        break;
+#define HRW_ADJ_CHUNK(KK, IN)                                                                    \
+    chunk_adjoint<KK, R, ALPHAMAX, IN>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,  \
+                                        a.xmin[line], a.rx[line], apart, line, off, first, nc,   \
+                                        &cand)
+    if ((long long)W - R - (K - 1) >= 0 && (long long)W + 63LL * R + (K - 1) <= n) {
+        if (K == 1) ab = HRW_ADJ_CHUNK(1, true);
+        else if (K == 2) ab = HRW_ADJ_CHUNK(2, true);
+        else ab = HRW_ADJ_CHUNK(3, true);
+    } else {
+        if (K == 1) ab = HRW_ADJ_CHUNK(1, false);
+        else if (K == 2) ab = HRW_ADJ_CHUNK(2, false);
+        else ab = HRW_ADJ_CHUNK(3, false);
     }
+#undef HRW_ADJ_CHUNK
     if constexpr (ALPHAMAX) {
         const double wsum = wave_sum(ab);
         const unsigned wmin = wave_umin(cand);
@@ -494,7 +504,7 @@ void adjoint_stage_kernel(AdjArgs a)
 }
 
 // Apply a pending correction to a final output: one wave per line.
-__global__ __launch_bounds__(64) void adjoint_fixup_kernel(Pending p, double *out,
+static __global__ __launch_bounds__(64) void adjoint_fixup_kernel(Pending p, double *out,
                                                            const long long *off, const double *h,
                                                            const int *line_chunk0,
                                                            const int *line_nchunks, int nlines)

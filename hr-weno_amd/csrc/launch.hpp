@@ -1,0 +1,42 @@
+// launch.hpp -- host launchers of the stage kernels, each compiled in a translation unit of
+// its own (they instantiate most of the device code) and called by the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/hrweno_capi.h"
+#include "kernels_1d.hpp"
+#include "kernels_1d_fast.hpp"
+#include "kernels_2d.hpp"
+#include "kernels_adjoint.hpp"
+
+namespace hrw {
+
+// Compute units of the current device (asked once).
+inline int device_cus()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// Register-blocked 1D stage (FAST form): r = cells per lane (4, 6, 8), negative for the
+// persistent prefetching kernel.  One definition per K (launch_fast_k<K>.o).
+void launch_fast_1d_k1(int flux, bool periodic, bool alphamax, int r, const Stage1DArgs &a, hipStream_t s);
+void launch_fast_1d_k2(int flux, bool periodic, bool alphamax, int r, const Stage1DArgs &a, hipStream_t s);
+void launch_fast_1d_k3(int flux, bool periodic, bool alphamax, int r, const Stage1DArgs &a, hipStream_t s);
+// LDS-tile 1D stage (EXACT form, and the FAST LDS variant).
+void launch_tile_1d(int k, int flux, bool periodic, bool fast, bool alphamax, const Stage1DArgs &a,
+                    hipStream_t s);
+// 2D population balance stage.
+void launch_pbe2d(int k, int bc, bool fast, const Stage2DArgs &a, hipStream_t s);
+// One stage of every line of a batch.
+void launch_batch_stage(int flux, bool alphamax, int R, const BatchArgs &a, hipStream_t s);
+// The reverse of one stage.
+void launch_adjoint_stage(int R, bool alphamax, const AdjArgs &g, hipStream_t s);
+
+}  // namespace hrw

@@ -232,7 +232,7 @@ class Problem:
         call("hrw_problem_neq", self._h, C.byref(n))
         return n.value
 
-    def tune(self, variant: int = 0, cells_per_lane: int = 8):
+    def tune(self, variant: int = 0, cells_per_lane: int = 0):
         """Kernel choice of the 1D FAST form (0: register-blocked, 1: LDS tile, 2: persistent);
         for a 2D problem, variant 0 and cells_per_lane = rows per wavefront (0: automatic)."""
         call("hrw_problem_tune", self._h, int(variant), int(cells_per_lane))
@@ -431,11 +431,13 @@ class Adjoint:
     """Discrete adjoint of ``nsteps`` fixed rktvd steps of periodic Lax-Friedrichs Burgers:
     one line (a :class:`Burgers1D` problem, any order, step ``dt``) or every line of a
     :class:`BurgersBatch` (RK3, each line's own dt).  ``forward`` records u0 -> u_N with a
-    checkpoint every ``ckpt_every`` steps; ``backward`` maps dJ/du_N to dJ/du_0.  The reference
-    has no adjoint (this is the added differentiation form of src/hrweno_tvdode.f90:136-171)."""
+    checkpoint every ``ckpt_every`` steps (0: the largest interval that fits in half of the free
+    device memory, usually the whole run, so nothing is recomputed); ``backward`` maps dJ/du_N
+    to dJ/du_0.  The reference has no adjoint (this is the added differentiation form of
+    src/hrweno_tvdode.f90:136-171)."""
 
     def __init__(self, system, nsteps: int, dt: Optional[float] = None, order: int = 3,
-                 ckpt_every: int = 20):
+                 ckpt_every: int = 0):
         h = C.c_void_p()
         self._system = system   # keeps the problem / batch alive
         self.batch = isinstance(system, BurgersBatch)

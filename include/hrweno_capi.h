@@ -153,7 +153,8 @@ int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, d
                       double x2min, double x2max, double g1a, double g1b, double g2a,
                       double g2b, int arith, hrw_problem_t *out);
 /* Kernel choice for the 1D FAST form (no reference counterpart; used by the autotuner):
- * variant 0 = register-blocked kernel with cells_per_lane (4 or 8) consecutive cells per lane,
+ * variant 0 = register-blocked kernel with cells_per_lane (4, 6 or 8; 0 = chosen per stage,
+ * the default) consecutive cells per lane,
  * variant 1 = LDS-tile kernel, variant 2 = the register-blocked kernel run persistently with
  * the next chunk's loads issued ahead of the current chunk's arithmetic.  Variants 0 and 2
  * give identical results; variant 1 agrees within the FAST tolerance.  For a 2D problem
@@ -221,7 +222,9 @@ int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane);
  * checkpoint every ckpt_every steps; backward maps a cotangent dJ/du_N to dJ/du_0,
  * recomputing one segment of ckpt_every steps at a time.  With alpha = max|u| the gradient
  * takes the subgradient at the first cell attaining the maximum.  Device memory:
- * (nsteps/ckpt_every + 1) + (order*ckpt_every - 1) + 2 states. */
+ * (nsteps/ckpt_every + 1) + (order*ckpt_every - 1) + 2 states.  ckpt_every = 0 chooses the
+ * largest interval whose states fit in half of the free device memory; when every stage input
+ * of the run fits (ckpt_every = nsteps), the backward recomputes nothing. */
 int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps, int ckpt_every,
                        hrw_adjoint_t *out);
 /* Every line of a batch (RK3, each line with its own dt); runs on the batch's stream and

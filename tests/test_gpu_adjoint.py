@@ -59,8 +59,10 @@ def test_checkpoint_interval_is_invisible(hh, dev):
     u0, w, dt = _case(n, 4)
     gN = _dev(np.sin(np.arange(n) * 0.1), dev)
     outs = []
-    for ck in (1, 5, 23, 100):
+    for ck in (1, 5, 23, 100, 0):
         adj = hh.Adjoint(hh.Burgers1D(n), nsteps=steps, dt=dt, ckpt_every=ck)
+        if ck == 0:   # automatic: the whole run fits, every stage input is kept
+            assert adj.info()["ckpt_every"] == steps
         adj.forward(_dev(u0, dev))
         outs.append(adj.backward(gN).cpu().numpy())
         # a second backward of the same forward (segments recomputed again) is identical
@@ -203,7 +205,7 @@ def test_adjoint_argument_errors(hh):
     with pytest.raises(hh.HrwError, match="periodic"):
         hh.Adjoint(hh.Burgers1D(100, bc=hh.BC_ZERO_GRADIENT), nsteps=3, dt=1e-3)
     with pytest.raises(hh.HrwError, match="ckpt_every"):
-        hh.Adjoint(hh.Burgers1D(100), nsteps=3, dt=1e-3, ckpt_every=0)
+        hh.Adjoint(hh.Burgers1D(100), nsteps=3, dt=1e-3, ckpt_every=-1)
     adj = hh.Adjoint(hh.Burgers1D(100), nsteps=3, dt=1e-3)
     import torch
     with pytest.raises(hh.HrwError, match="recorded"):

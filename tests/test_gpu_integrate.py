@@ -76,11 +76,13 @@ def test_tiling_invariance_exact(oracle, hh, dev, n):
 @pytest.mark.parametrize("bc", [0, 1])
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
-    """FAST form through the 1D kernels (register-blocked with 4 or 8 cells per lane, its
-    persistent prefetching form, and the LDS tile) on line lengths that hit every
-    edge/interior case: each within the FAST tolerance of the oracle, and the register-blocked
-    forms bit-identical (the per-cell arithmetic does not depend on which lane computes it)."""
-    for n in (1, 2, 5, 37, 255, 256, 2049, 2050, 4093, 20000, 300001):
+    """FAST form through the 1D kernels (register-blocked with 4, 6 or 8 cells per lane or
+    chosen per stage, its persistent prefetching form, and the LDS tile) on line lengths that
+    hit every edge/interior case (chunks of 62R output cells with R + K - 1 halo cells before
+    and 63R + K - 1 after): each within the FAST tolerance of the oracle, and the
+    register-blocked forms bit-identical (the per-cell arithmetic does not depend on which
+    lane computes it)."""
+    for n in (1, 2, 5, 37, 247, 248, 255, 256, 378, 507, 1000, 2049, 2050, 4093, 20000, 300001):
         rng = np.random.default_rng(1000 * k + n)
         u0 = 0.4 + rng.standard_normal(n) * 0.3 + (np.arange(n) > n // 3)
         e = -1.0 + (2.0 / n) * np.arange(n + 1)
@@ -89,7 +91,7 @@ def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
         ou, ot, steps, fe = oracle.burgers_rk(u0, w, k=k, bc=bc, flux=0 if bc == 0 else 1,
                                               tout=5.5 * dt, dt=dt)
         res = {}
-        for variant, r in ((0, 8), (0, 4), (2, 4), (1, 8)):
+        for variant, r in ((0, 8), (0, 4), (0, 6), (0, 0), (2, 4), (2, 6), (2, 8), (1, 8)):
             p = hh.Burgers1D(n, k, 1e-6, flux=0 if bc == 0 else 1, bc=bc, arith="fast")
             p.tune(variant, r)
             ode = hh.RkTvd(p, order=3)
@@ -98,8 +100,8 @@ def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
             assert t == ot and ode.fevals == fe
             res[(variant, r)] = _h(u)
             assert_close(res[(variant, r)], ou)
-        assert np.array_equal(res[(0, 8)], res[(0, 4)]), n
-        assert np.array_equal(res[(0, 4)], res[(2, 4)]), n
+        for key in ((0, 4), (0, 6), (0, 0), (2, 4), (2, 6), (2, 8)):
+            assert np.array_equal(res[(0, 8)], res[key]), (n, key)
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
