@@ -1469,7 +1469,8 @@ struct hrw_adjoint_s {
     int *d_n = nullptr, *d_k = nullptr;
     double *d_h = nullptr, *d_xmin = nullptr, *d_rx = nullptr;
     bool own_lines = false;
-    int *d_cline = nullptr, *d_cstart = nullptr, *d_lchunk0 = nullptr, *d_lnch = nullptr;
+    AdjChunk *d_chunks = nullptr;
+    int *d_lchunk0 = nullptr, *d_lnch = nullptr;
     long long nchunks = 0;
     // alpha-bar bookkeeping of the reverse launches: launch r writes slot r % 4 (per-chunk
     // partial sums, per-line argmax keys tagged with r's tag); the correction it owes to its
@@ -1496,7 +1497,7 @@ struct hrw_adjoint_s {
         if (own_lines) {
             dfree(d_off); dfree(d_n); dfree(d_k); dfree(d_h); dfree(d_xmin); dfree(d_rx);
         }
-        dfree(d_cline); dfree(d_cstart); dfree(d_lchunk0); dfree(d_lnch);
+        dfree(d_chunks); dfree(d_lchunk0); dfree(d_lnch);
         for (auto &q : partial) dfree(q);
         for (auto &q : bpartial) dfree(q);
         for (auto &q : amax) dfree(q);
@@ -1539,29 +1540,52 @@ static int adjoint_alloc(hrw_adjoint_s *a, const std::vector<int> &ncells)
 // Chunk table of the reverse pass (output chunks of 62R cells) and the per-chunk slots.
 static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
 {
-    dfree(a->d_cline); dfree(a->d_cstart); dfree(a->d_lchunk0); dfree(a->d_lnch);
+    dfree(a->d_chunks); dfree(a->d_lchunk0); dfree(a->d_lnch);
     for (auto &q : a->partial) dfree(q);
     for (auto &q : a->bpartial) dfree(q);
     for (auto &q : a->amax) dfree(q);
     a->owed.clear();
-    std::vector<int> cl, cs, c0, nc;
+    std::vector<AdjChunk> ch;
+    std::vector<int> c0, nc;
     for (int l = 0; l < (int)ncells.size(); ++l) {
         // every line starts on a block of four chunks (idle chunks pad the gap), so its
         // alpha-bar is summed in the same order as for a line on its own
-        while (cl.size() % 4) {
-            cl.push_back(-1);
-            cs.push_back(0);
+        while (ch.size() % 4) {
+            AdjChunk idle{};
+            idle.line = -1;
+            ch.push_back(idle);
         }
-        c0.push_back((int)cl.size());
+        c0.push_back((int)ch.size());
+        AdjChunk c{};
+        c.line = l;
+        c.n = ncells[l];
+        if (a->b) {
+            c.off = a->b->h_off[l];
+            c.k = a->b->h_k[l];
+            c.h = a->b->h_dt[l];
+            c.xmin = a->b->h_xmin[l];
+            c.rx = a->b->h_rx[l];
+        } else {
+            c.off = 0;
+            c.k = a->p->k;
+            c.h = a->dt;
+            c.xmin = a->p->xmin;
+            c.rx = a->p->rx;
+        }
         for (long long st = 0; st < ncells[l]; st += 62LL * a->R) {
-            cl.push_back(l);
-            cs.push_back((int)st);
+            c.start = (int)st;
+            ch.push_back(c);
         }
-        nc.push_back((int)cl.size() - c0.back());
+        nc.push_back((int)ch.size() - c0.back());
     }
-    a->nchunks = (long long)cl.size();
-    CALL(upload(&a->d_cline, cl));
-    CALL(upload(&a->d_cstart, cs));
+    for (auto &c : ch) {
+        if (c.line >= 0) {
+            c.first = c0[(size_t)c.line];
+            c.nc = nc[(size_t)c.line];
+        }
+    }
+    a->nchunks = (long long)ch.size();
+    CALL(upload(&a->d_chunks, ch));
     CALL(upload(&a->d_lchunk0, c0));
     CALL(upload(&a->d_lnch, nc));
     const size_t L = ncells.size();
@@ -1794,8 +1818,7 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
     g.h = a->d_h;
     g.xmin = a->d_xmin;
     g.rx = a->d_rx;
-    g.chunk_line = a->d_cline;
-    g.chunk_start = a->d_cstart;
+    g.chunks = a->d_chunks;
     g.line_chunk0 = a->d_lchunk0;
     g.line_nchunks = a->d_lnch;
     g.nchunks = a->nchunks;

@@ -44,17 +44,45 @@
 
 namespace hrw {
 
+// Wavefront sum and minimum by DPP row shifts and row broadcasts (no LDS round trips), in one
+// fixed order: after the row shifts by 1, 2, 4, 8 each lane holds its row's prefix, lane 15 of
+// a row the row's total; row_bcast:15 and row_bcast:31 carry rows 0-2 up so that lane 63 holds
+// the wavefront's, which every lane gets.  Every lane of the wavefront must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64_or_zero(double v)   // no source: 0
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64_or_zero<0x111, 0xF>(v);
+    v += dpp_f64_or_zero<0x112, 0xF>(v);
+    v += dpp_f64_or_zero<0x114, 0xF>(v);
+    v += dpp_f64_or_zero<0x118, 0xF>(v);
+    v += dpp_f64_or_zero<0x142, 0xA>(v);
+    v += dpp_f64_or_zero<0x143, 0xC>(v);
+    const long long b = __builtin_bit_cast(long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | lo);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned dpp_u32_or_max(unsigned v)   // no source: 0xffffffff
+{
+    return (unsigned)__builtin_amdgcn_update_dpp((int)0xffffffffu, (int)v, CTRL, ROW_MASK, 0xF, false);
 }
 __device__ __forceinline__ unsigned wave_umin(unsigned v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, off, 64));
-    return v;
+    v = min(v, dpp_u32_or_max<0x111, 0xF>(v));
+    v = min(v, dpp_u32_or_max<0x112, 0xF>(v));
+    v = min(v, dpp_u32_or_max<0x114, 0xF>(v));
+    v = min(v, dpp_u32_or_max<0x118, 0xF>(v));
+    v = min(v, dpp_u32_or_max<0x142, 0xA>(v));
+    v = min(v, dpp_u32_or_max<0x143, 0xC>(v));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // The not yet applied alpha correction of an earlier launch's output.
@@ -67,6 +95,16 @@ struct Pending {
     unsigned tag;
 };
 
+// One wavefront's work in one 64-byte record, so that a single scalar load gives the chunk
+// and its line's parameters (instead of a load of the line index and then of each parameter).
+struct AdjChunk {
+    long long off;              // the line's first cell in the flat state
+    int line, start, n, k;      // line (-1: an idle padding chunk), first output cell, n, WENO k
+    int first, nc;              // the line's chunks [first, first + nc)
+    double h, xmin, rx, pad;
+};
+static_assert(sizeof(AdjChunk) == 64, "one 64-byte record per chunk");
+
 struct AdjArgs {
     const double *s;            // stage input state (flat, line l at off[l])
     const double *lam;          // cotangent of the stage output
@@ -75,7 +113,7 @@ struct AdjArgs {
     const long long *off;
     const int *n, *k;
     const double *h, *xmin, *rx;
-    const int *chunk_line, *chunk_start;
+    const AdjChunk *chunks;     // the chunk table
     const int *line_chunk0, *line_nchunks;
     long long nchunks;
     double a, c, e1, e2, eps, alpha;
@@ -117,7 +155,7 @@ __device__ __forceinline__ double wave_sum_array(const double *q, int count)
 #pragma unroll
         for (int u = 0; u < w; ++u) acc[u] += acc[u + w];
     }
-    return __shfl(wave_sum(acc[0]), 0, 64);
+    return wave_sum(acc[0]);
 }
 
 // The line's alpha-bar: the chunks [first, first + nc) are summed as the chunks before the
@@ -448,17 +486,18 @@ void adjoint_stage_kernel(AdjArgs a)
     const long long wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
     const int lane = threadIdx.x & 63;
     __shared__ double s_part[kBlock / 64];
-    if (wave >= a.nchunks || a.chunk_line[wave] < 0) {   // idle chunks only join the block sum
+    const AdjChunk &ch = a.chunks[wave < a.nchunks ? wave : 0];
+    if (wave >= a.nchunks || ch.line < 0) {   // idle chunks only join the block sum
         if constexpr (ALPHAMAX) {
             if (lane == 0) s_part[threadIdx.x >> 6] = 0.0;
             __syncthreads();
         }
         return;
     }
-    const int line = a.chunk_line[wave];
-    const long long off = a.off[line];
-    const int n = a.n[line];
-    const double h = a.h[line];
+    const int line = ch.line;
+    const long long off = ch.off;
+    const int n = ch.n;
+    const double h = ch.h;
     // the alpha partials (reduced inside the chunk, after its loads are issued)
     unsigned long long apart = 0ull;
     if constexpr (ALPHAMAX) {
@@ -468,14 +507,14 @@ void adjoint_stage_kernel(AdjArgs a)
     }
     const double *x1 = a.x1 ? a.x1 + off : nullptr;
     const double *x2 = a.x2 ? a.x2 + off : nullptr;
-    const int first = a.line_chunk0[line], nc = a.line_nchunks[line];
-    const int W = a.chunk_start[wave];
-    const int K = a.k[line];
+    const int first = ch.first, nc = ch.nc;
+    const int W = ch.start;
+    const int K = ch.k;
     unsigned cand;
     double ab;
 #define HRW_ADJ_CHUNK(KK, IN)                                                                    \
     chunk_adjoint<KK, R, ALPHAMAX, IN>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,  \
-                                        a.xmin[line], a.rx[line], apart, line, off, first, nc,   \
+                                        ch.xmin, ch.rx, apart, line, off, first, nc,             \
                                         &cand)
     if ((long long)W - R - (K - 1) >= 0 && (long long)W + 63LL * R + (K - 1) <= n) {
         if (K == 1) ab = HRW_ADJ_CHUNK(1, true);
