@@ -132,16 +132,15 @@ def run_plain(args, torch, hh, W, dev):
         raise RuntimeError("non-finite state after the timed steps")
     log(f"timed {args.steps} steps: {ms:.3f} ms")
     fast_v2 = args.arith == "fast" and args.variant == 0
-    cpl = (str(args.cells_per_lane) if args.cells_per_lane
-           else "R,NPW> with <R,NPW> = <8,0> for stage 1 and <4,1> for stages 2-3")
-    kernel = ("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%s" % cpl
-              + (">" if args.cells_per_lane else "")
+    cpl = args.cells_per_lane or (6 if c.n >= (1 << 19) else 4)   # the library's automatic choice
+    kernel = ("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%d,NPW> (NPW = 0 for stage 1, 1 for "
+              "stages 2-3)" % cpl
               if fast_v2 else "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith)
     rl = roofline(64.0 / 3 * c.n, ms / (3 * args.steps), kernel, "stage1d_2p24")
     config = {"workload": "burgers1d_weno5_lf_rk3_periodic_2^24cells", "cells": c.n, "k": 3,
               "eps": 1e-6, "flux": "lax_friedrichs, alpha=max|u| per stage",
               "integrator": "rktvd order 3", "cfl": 0.4, "dt": c.dt, "arith": args.arith,
-              "variant": args.variant, "cells_per_lane": args.cells_per_lane,
+              "variant": args.variant, "cells_per_lane": cpl,
               "parallelism": "single GPU"}
     return c.n * args.steps / (ms / 1e3), ms / args.steps, rl, config, \
         "synthetic (seeded SplitMix64: 8 sine modes + 16 jumps)"
@@ -318,7 +317,7 @@ def main() -> int:
     ap.add_argument("--variant", type=int, default=0,
                     help="1D FAST kernel: 0 register-blocked, 1 LDS tile")
     ap.add_argument("--cells-per-lane", type=int, default=0,
-                    help="cells per lane of the 1D stage kernel (0: by stage)")
+                    help="cells per lane of the 1D stage kernel (0: chosen by the line length)")
     ap.add_argument("--per-line-steps", type=int, default=5)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of --cells doubles (20 times), the "

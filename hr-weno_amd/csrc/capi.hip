@@ -716,9 +716,9 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
         const bool per = p->bc == HRW_BC_PERIODIC;
-        // cells per lane: tuned, or by the stage's operand count (a stage that reads u0 as well
-        // is load-bound, so smaller chunks keep more loads in flight per compute unit)
-        const int cpl = p->cells_per_lane ? p->cells_per_lane : (stage_npw(a.stage) == 0 ? 8 : 4);
+        // cells per lane: tuned, or 6 (measured best on 2^20..2^24 cells), 4 on short lines
+        // (more wavefronts to hide the load latency)
+        const int cpl = p->cells_per_lane ? p->cells_per_lane : (a.n >= (1LL << 19) ? 6 : 4);
         const int r = p->variant == 2 ? -cpl : cpl;   // < 0: persistent
         if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, r, a, s);
         else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, r, a, s);
