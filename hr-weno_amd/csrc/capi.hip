@@ -1473,12 +1473,13 @@ struct hrw_adjoint_s {
     int *d_lchunk0 = nullptr, *d_lnch = nullptr;
     long long nchunks = 0;
     // alpha-bar bookkeeping of the reverse launches: launch r writes slot r % 4 (per-chunk
-    // partial sums, per-line argmax keys tagged with r's tag); the correction it owes to its
-    // output is applied by the launch that next reads that buffer (at most 3 launches later),
-    // or by a fixup at the end of the backward.
+    // partial sums, per-line records with the argmax key tagged with r's tag); the correction
+    // it owes to its output is applied by the launches that read that buffer (at most 3
+    // launches later; the first of them records it in the line records for the others), or by
+    // a fixup at the end of the backward.
     static constexpr int kSlots = 4;
     double *partial[kSlots] = {}, *bpartial[kSlots] = {};
-    unsigned long long *amax[kSlots] = {};
+    LineRec *recs = nullptr;      // kSlots x nlines
     unsigned tag = 0;
     int64_t launches = 0;
     std::vector<std::pair<const double *, Pending>> owed;   // buffer -> its pending correction
@@ -1500,13 +1501,33 @@ struct hrw_adjoint_s {
         dfree(d_chunks); dfree(d_lchunk0); dfree(d_lnch);
         for (auto &q : partial) dfree(q);
         for (auto &q : bpartial) dfree(q);
-        for (auto &q : amax) dfree(q);
+        dfree(recs);
     }
     Pending owed_by(const double *buf) const
     {
         for (auto &e : owed)
             if (e.first == buf) return e.second;
         return Pending{};
+    }
+    // after a launch that read buf: its correction is recorded per line
+    void mark_resolved(const double *buf)
+    {
+        for (auto &e : owed)
+            if (e.first == buf && e.second.partial) e.second.ready = 1;
+    }
+    // buf's pending correction, recorded per line (by a resolve launch if no launch has read
+    // buf as lam yet)
+    Pending resolved_owed(const double *buf)
+    {
+        Pending p = owed_by(buf);
+        if (p.partial && !p.ready) {
+            adjoint_resolve_kernel<<<nlines(), 64, 0, strm()>>>(p, d_off, d_h, d_lchunk0, d_lnch,
+                                                                nlines());
+            // (a launch error surfaces at the stage launch's check that follows)
+            mark_resolved(buf);
+            p.ready = 1;
+        }
+        return p;
     }
     void set_owed(const double *buf, const Pending &p)
     {
@@ -1543,7 +1564,7 @@ static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
     dfree(a->d_chunks); dfree(a->d_lchunk0); dfree(a->d_lnch);
     for (auto &q : a->partial) dfree(q);
     for (auto &q : a->bpartial) dfree(q);
-    for (auto &q : a->amax) dfree(q);
+    dfree(a->recs);
     a->owed.clear();
     std::vector<AdjChunk> ch;
     std::vector<int> c0, nc;
@@ -1591,10 +1612,13 @@ static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
     const size_t L = ncells.size();
     for (int q = 0; q < hrw_adjoint_s::kSlots; ++q) {
         CALL(dalloc(&a->partial[q], (size_t)a->nchunks));
-        CALL(dalloc(&a->bpartial[q], (size_t)(a->nchunks + 3) / 4));
-        CALL(dalloc(&a->amax[q], L));
-        fill_u64_kernel<<<grid_for((long long)L, 256, 4096), 256, 0, a->strm()>>>(a->amax[q], (long long)L);
+        // (padded: the alpha-bar sums load whole rounds past a line's last block)
+        CALL(dalloc(&a->bpartial[q], (size_t)(a->nchunks + 3) / 4 + kBlockPad));
     }
+    CALL(dalloc(&a->recs, (size_t)hrw_adjoint_s::kSlots * L));
+    const long long words = 2LL * hrw_adjoint_s::kSlots * (long long)L;
+    fill_u64_kernel<<<grid_for(words, 256, 4096), 256, 0, a->strm()>>>(
+        reinterpret_cast<unsigned long long *>(a->recs), words);
     LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(a->strm()));
     return HRW_OK;
@@ -1812,15 +1836,7 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
     g.x1 = x1;
     g.x2 = x2;
     g.out = out;
-    g.off = a->d_off;
-    g.n = a->d_n;
-    g.k = a->d_k;
-    g.h = a->d_h;
-    g.xmin = a->d_xmin;
-    g.rx = a->d_rx;
     g.chunks = a->d_chunks;
-    g.line_chunk0 = a->d_lchunk0;
-    g.line_nchunks = a->d_lnch;
     g.nchunks = a->nchunks;
     g.a = ca;
     g.c = cc;
@@ -1835,26 +1851,37 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
                   : reinterpret_cast<const unsigned long long *>(a->alpha_rec + q);
     const bool amax = a->alphamax();
     if (amax) {
-        if (a->tag >= 0xfffffff0u) {   // tags wrap: start over from clean slots
-            for (auto *q : a->amax)
-                fill_u64_kernel<<<grid_for(a->nlines(), 256, 4096), 256, 0, a->strm()>>>(q, a->nlines());
+        if (a->tag >= kNoPendTag - 16) {   // tags wrap: start over from clean slots
+            const long long words = 2LL * hrw_adjoint_s::kSlots * a->nlines();
+            fill_u64_kernel<<<grid_for(words, 256, 4096), 256, 0, a->strm()>>>(
+                reinterpret_cast<unsigned long long *>(a->recs), words);
             a->tag = 0;
         }
         const int slot = (int)(a->launches % hrw_adjoint_s::kSlots);
         g.partial_out = a->partial[slot];
         g.bpartial_out = a->bpartial[slot];
-        g.amax_out = a->amax[slot];
+        g.rec_out = a->recs + (size_t)slot * a->nlines();
         g.tag = ++a->tag;
-        g.pend[0] = a->owed_by(lam);
-        g.pend[1] = x1 ? a->owed_by(x1) : Pending{};
-        g.pend[2] = x2 ? a->owed_by(x2) : Pending{};
+        const Pending owed[3] = {a->owed_by(lam), x1 ? a->resolved_owed(x1) : Pending{},
+                                 x2 ? a->resolved_owed(x2) : Pending{}};
+        a->mark_resolved(lam);
+        g.recs = a->recs;
+        g.nlines = a->nlines();
+        g.pend_slots = 0;
+        for (int i = 0; i < 3; ++i) {
+            // none owed: slot 0's records (the kernel reads them) with a tag never used
+            g.pend_slots |= (unsigned)(owed[i].partial ? owed[i].slot : 0) << (4 * i);
+            g.pend_tag[i] = owed[i].partial ? owed[i].tag : kNoPendTag;
+        }
+        g.lam_pend = owed[0];
         Pending mine{};
         mine.partial = g.partial_out;
         mine.bpartial = g.bpartial_out;
-        mine.amax = g.amax_out;
+        mine.rec = g.rec_out;
         mine.s = g.s;
         mine.cf = cc;
         mine.tag = g.tag;
+        mine.slot = slot;
         a->set_owed(out, mine);   // out's earlier debt (if out aliases x1/x2) is settled here
         ++a->launches;
     }

@@ -32,10 +32,14 @@
 // cell) is known only when the launch has finished.  Each wave writes its partial sum to
 // partial_out[c] (and each block the sum of its four, in order, to bpartial_out[b]) and, if it
 // holds the argmax, atomicMax-es the key (tag << 32 | ~cell) into
-// amax_out[line] (tags grow with every launch, so no slot is ever reset and the smallest cell
+// rec_out[line].key (tags grow with every launch, so no slot is ever reset and the smallest cell
 // wins).  The launch that next reads this output (as lam, x1 or x2) gets it as a Pending and
 // adds the correction  cf*h * sum(partials) * sign(s_prev[cell])  to the value it loads at
-// that cell, in registers; every wave that needs it sums the partials in the same order.
+// that cell, in registers.  A launch reading it as lam sums the partials (every wave that needs
+// the sum, in the same order) and records the correction in rec[line].resolved; a launch
+// reading it as x1 or x2 loads that (recorded by an earlier launch, or by
+// adjoint_resolve_kernel when none has read the buffer as lam).
+// The host keeps four slots of these arrays and the launches write them in turn.
 // No wave ever waits for another, and no memory fence is needed inside a launch.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -85,14 +89,26 @@ __device__ __forceinline__ unsigned wave_umin(unsigned v)
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// The not yet applied alpha correction of an earlier launch's output.
+// A launch's record per line: its argmax key, and the correction it owes at that cell once a
+// later launch has summed it.
+struct LineRec {
+    unsigned long long key;
+    double resolved;
+};
+static_assert(sizeof(LineRec) == 16, "one 16-byte record per line");
+
+// The not yet applied alpha correction of an earlier launch's output.  A tag of kNoPendTag
+// (which no launch uses) stands for none owed.
+constexpr unsigned kNoPendTag = 0xffffffffu;
 struct Pending {
     const double *partial;              // that launch's per-chunk partial sums (null: none)
     const double *bpartial;             // ... and per-block sums of its four chunks
-    const unsigned long long *amax;     // its per-line argmax keys
+    LineRec *rec;                       // its per-line records
     const double *s;                    // its stage input (for the sign at the argmax)
     double cf;                          // its c (the correction is cf * h[line] * sum)
     unsigned tag;
+    int ready;                          // rec[].resolved was written by an earlier launch
+    int slot;                           // the slot it wrote
 };
 
 // One wavefront's work in one 64-byte record, so that a single scalar load gives the chunk
@@ -106,97 +122,106 @@ struct AdjChunk {
 static_assert(sizeof(AdjChunk) == 64, "one 64-byte record per chunk");
 
 struct AdjArgs {
-    const double *s;            // stage input state (flat, line l at off[l])
+    const double *s;            // stage input state (flat, line l at chunk.off)
     const double *lam;          // cotangent of the stage output
     const double *x1, *x2;      // pointwise cotangents folded into out (or null)
     double *out;                // may alias x1 or x2 (read and written by the same lane)
-    const long long *off;
-    const int *n, *k;
-    const double *h, *xmin, *rx;
     const AdjChunk *chunks;     // the chunk table
-    const int *line_chunk0, *line_nchunks;
     long long nchunks;
     double a, c, e1, e2, eps, alpha;
     const unsigned long long *aset;   // alpha partials: line l, partial p at
     int aline_stride, anpart, apart_stride;   //   aset[l*aline_stride + p*apart_stride]
     double *partial_out;        // this launch: one partial alpha-bar per chunk
     double *bpartial_out;       // this launch: their sums per block of four chunks
-    unsigned long long *amax_out;     // this launch: argmax key per line
+    LineRec *rec_out;           // this launch: its records (argmax key per line)
     unsigned tag;
-    Pending pend[3];            // corrections owed by lam, x1, x2
+    // the corrections owed by lam, x1 and x2: the slot each was written to (bits 4i, 4i+1; the
+    // records of slot q at recs[q * nlines]) and its tag (kNoPendTag: none owed)
+    const LineRec *recs;
+    int nlines;
+    unsigned pend_slots;
+    unsigned pend_tag[3];
+    Pending lam_pend;           // lam's, to sum it here
 };
 
 // The cell a pending correction applies to on `line` (wave-uniform), or -1 if none.
 __device__ __forceinline__ int pending_cell(const Pending &p, int line)
 {
     if (!p.partial) return -1;
-    const unsigned long long key = p.amax[line];
+    const unsigned long long key = p.rec[line].key;
     if ((unsigned)(key >> 32) != p.tag) return -1;
     return (int)(0xffffffffu - (unsigned)(key & 0xffffffffull));
 }
-// Sum of q[0..count) by one wave in a fixed order: 16 independent loads in flight per lane
-// per round (1024 values per round).
-__device__ __forceinline__ double wave_sum_array(const double *q, int count)
+// The line's alpha-bar from an earlier launch: the whole blocks of four chunks [b0, b1) by
+// their sums, and the chunks outside them ([first, 4 b0) and [4 b1, end), or all of the line's
+// chunks when it holds no whole block) one by one.  Lane l adds the block sums l, l + 64,
+// l + 128, ... in that order, the wavefront adds the 64 lane sums (wave_sum), and then the
+// single chunks (at most six, one per lane) the same way.  That order does not depend on U,
+// the loads a lane has in flight at once, so every wave that needs the sum gets the same bits.
+// Loads past b1 stay inside the block-sum array (it holds kBlockPad more) and are not counted.
+constexpr int kBlockPad = 64 * 40;
+template <int U>
+__device__ __forceinline__ double line_alpha_bar(const Pending &p, int first, int nc)
 {
+    static_assert(64 * U <= kBlockPad, "block-sum array padding");
     const int lane = threadIdx.x & 63;
-    double acc[16];
+    const int end = first + nc;
+    const int b0 = (first + 3) >> 2, b1 = end >> 2;
+    const int nb = b1 > b0 ? b1 - b0 : 0;
+    const int e0 = nb ? 4 * b0 : end, e1 = nb ? 4 * b1 : end;   // singles [first, e0), [e1, end)
+    const int hh = e0 - first, tt = end - e1;
+    double single = 0.0;
+    if (lane < hh) single = p.partial[first + lane];
+    else if (lane < hh + tt) single = p.partial[e1 + (lane - hh)];
+    const double *q = p.bpartial + b0;
+    double acc = 0.0;
+    for (int base = 0; base < nb; base += 64 * U) {
+        double v[U];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-    int c = lane;
-    for (; c + 15 * 64 < count; c += 16 * 64) {
+        for (int u = 0; u < U; ++u) v[u] = q[base + 64 * u + lane];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) acc[u] += q[c + u * 64];
+        for (int u = 0; u < U; ++u) acc += base + 64 * u + lane < nb ? v[u] : 0.0;
     }
-#pragma unroll
-    for (int u = 0; u < 16; ++u)
-        if (c + u * 64 < count) acc[u] += q[c + u * 64];
-#pragma unroll
-    for (int w = 8; w > 0; w >>= 1) {
-#pragma unroll
-        for (int u = 0; u < w; ++u) acc[u] += acc[u + w];
-    }
-    return wave_sum(acc[0]);
+    return wave_sum(acc) + wave_sum(single);
 }
 
-// The line's alpha-bar: the chunks [first, first + nc) are summed as the chunks before the
-// first whole block of four, the whole blocks' sums, and the chunks after the last one.
+// The correction owed at cell j of a line (whole wave), with up to UMAX block sums per lane in
+// flight: about nc/4 block sums, so one round of loads for lines of up to 16 UMAX blocks of four
+// chunks (1.27M cells at UMAX = 40 and 2 cells per lane).
+template <int UMAX>
 __device__ __forceinline__ double pending_delta(const Pending &p, int j, long long off, double h,
                                                 int first, int nc)
 {
-    const int end = first + nc;
-    const int b0 = (first + 3) >> 2, b1 = end >> 2;   // whole blocks [b0, b1)
-    double sum;
-    if (b1 <= b0) {
-        sum = wave_sum_array(p.partial + first, nc);
-    } else {
-        double head = 0.0, tail = 0.0;
-        for (int c = first; c < 4 * b0; ++c) head += p.partial[c];
-        for (int c = 4 * b1; c < end; ++c) tail += p.partial[c];
-        sum = (head + wave_sum_array(p.bpartial + b0, b1 - b0)) + tail;
-    }
     const double sj = p.s[off + j];
+    const double sum = (nc >> 2) <= 512 ? line_alpha_bar<8>(p, first, nc)
+                                        : line_alpha_bar<UMAX>(p, first, nc);
     return p.cf * h * sum * (double)((sj > 0.0) - (sj < 0.0));
 }
 
-// Add a pending correction to the N values vals[i] a lane loaded from cells cell[i] (only
-// where `active`).  Must be called by the whole wave.
-template <int N>
-__device__ __forceinline__ void apply_pending(const Pending &p, int line, long long off, double h,
-                                              int first, int nc, const long long *cell,
-                                              bool active, double *vals)
+// A pending correction's cell on a line (-1: none) and its value if an earlier launch resolved
+// it.  Lanes 0, 1, 2 load the records of lam's, x1's and x2's corrections (one vector load,
+// issued before the chunk's own loads, so that waiting for it never waits for those), and
+// each is taken from its lane where it is used.
+struct PendAt {
+    int j;
+    double d;
+};
+__device__ __forceinline__ LineRec pend_recs(const AdjArgs &a, int line)
 {
-    const int j = pending_cell(p, line);
-    if (j < 0) return;
-    bool hit = false;
-#pragma unroll
-    for (int i = 0; i < N; ++i) hit = hit || (active && cell[i] == j);
-    if (!__any(hit)) return;
-    const double d = pending_delta(p, j, off, h, first, nc);
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-        if (active && cell[i] == j) vals[i] += d;
+    const int i = min((int)(threadIdx.x & 63), 2);
+    const int slot = (int)(a.pend_slots >> (4 * i)) & 3;
+    return a.recs[(long long)slot * a.nlines + line];
 }
-
+__device__ __forceinline__ PendAt pend_at(unsigned tag, const LineRec &r, int i)
+{
+    const unsigned klo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)r.key, i);
+    const unsigned khi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(r.key >> 32), i);
+    const long long rb = __builtin_bit_cast(long long, r.resolved);
+    const unsigned rlo = (unsigned)__builtin_amdgcn_readlane((int)rb, i);
+    const unsigned rhi = (unsigned)__builtin_amdgcn_readlane((int)(rb >> 32), i);
+    const double d = __builtin_bit_cast(double, ((unsigned long long)rhi << 32) | rlo);
+    return PendAt{khi == tag ? (int)(0xffffffffu - klo) : -1, d};
+}
 
 // Reverse of one K = 3 reconstruction of cell i from its window x[0..4] = v_{i-2..i+2}, for
 // the flux cotangents phi3 = (phi_{i-1}, phi_i, phi_{i+1}).  The triple pieces are rebuilt
@@ -311,7 +336,7 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
                                                 const double *lam, const double *x1,
                                                 const double *x2, double *out, int n, int W,
                                                 double h, double xmin, double rx,
-                                                unsigned long long alpha_part,
+                                                unsigned long long alpha_part, LineRec prec,
                                                 int line, long long off, int first, int nc,
                                                 unsigned *cand)
 {
@@ -376,7 +401,48 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     long long cell[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) cell[i] = interior ? c0 + i : cell_index(c0 + i, n, true);
-    apply_pending<R>(a.pend[0], line, off, h, first, nc, cell, true, mu);
+    // lanes 0 and 63 also form the phi just outside their cells themselves: with R < 2K-1
+    // their reconstructions reach the owned cells of lanes 1 and 62
+    const bool edge = lane == 0 || lane == 63;
+    const long long hc = interior ? (lane == 0 ? c0 - 1 : c0 + R) : cell_index(lane == 0 ? c0 - 1 : c0 + R, n, true);
+    // the pending corrections: lam's at any loaded cell, x1's and x2's (resolved by an earlier
+    // launch) at the owned cells; only the alpha-max form owes any
+    const PendAt none{-1, 0.0};
+    const PendAt q0 = ALPHAMAX ? pend_at(a.pend_tag[0], prec, 0) : none;
+    const PendAt q1 = ALPHAMAX ? pend_at(a.pend_tag[1], prec, 1) : none;
+    const PendAt q2 = ALPHAMAX ? pend_at(a.pend_tag[2], prec, 2) : none;
+    bool h0 = edge && hc == q0.j, h1 = false, h2 = false;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        h0 = h0 || cell[i] == q0.j;
+        h1 = h1 || (owned && cell[i] == q1.j);
+        h2 = h2 || (owned && cell[i] == q2.j);
+    }
+    const bool hit0 = ALPHAMAX && q0.j >= 0 && __any(h0);
+    const bool hit1 = ALPHAMAX && q1.j >= 0 && __any(h1);
+    const bool hit2 = ALPHAMAX && q2.j >= 0 && __any(h2);
+    if (hit0) {
+        // summed here (every wave that sums it writes the same bits) unless resolved earlier
+        double d = q0.d;
+        if (!a.lam_pend.ready) {
+            // lam_pend read from the kernel arguments here, where it is needed, rather than
+            // held in registers from the kernel's start
+            const auto *kp = (const __attribute__((address_space(4))) AdjArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+            Pending lp;
+            lp.partial = kp->lam_pend.partial;
+            lp.bpartial = kp->lam_pend.bpartial;
+            lp.rec = kp->lam_pend.rec;
+            lp.s = kp->lam_pend.s;
+            lp.cf = kp->lam_pend.cf;
+            // (fewer block sums in flight with 4 cells per lane, which need more registers)
+            d = pending_delta<R <= 2 ? 40 : 16>(lp, q0.j, off, h, first, nc);
+            if (lane == 0) lp.rec[line].resolved = d;
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (cell[i] == q0.j) mu[i] += d;
+        if (edge && hc == q0.j) lam_edge += d;
+    }
     // phi = lam / width for the lane's cells, neighbours' from the adjacent lanes
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -386,20 +452,11 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     }
     phi[0] = __shfl_up(phi[R], 1, 64);
     phi[R + 1] = __shfl_down(phi[1], 1, 64);
-    {
-        // lanes 0 and 63 form the phi just outside their cells themselves: with R < 2K-1 their
-        // reconstructions reach the owned cells of lanes 1 and 62
-        const bool edge = lane == 0 || lane == 63;
-        const long long pos = lane == 0 ? c0 - 1 : c0 + R;
-        long long hc[1] = {interior ? pos : cell_index(pos, n, true)};
-        double hv[1] = {lam_edge};
-        apply_pending<1>(a.pend[0], line, off, h, first, nc, hc, edge, hv);
-        if (edge) {
-            const double w = linear_edge(xmin, rx, hc[0] + 1) - linear_edge(xmin, rx, hc[0]);
-            const double ph = hv[0] * rcp_nr(w);
-            if (lane == 0) phi[0] = ph;
-            else phi[R + 1] = ph;
-        }
+    if (edge) {
+        const double w = linear_edge(xmin, rx, hc + 1) - linear_edge(xmin, rx, hc);
+        const double ph = lam_edge * rcp_nr(w);
+        if (lane == 0) phi[0] = ph;
+        else phi[R + 1] = ph;
     }
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = 0.0;
@@ -452,8 +509,16 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     }
     // out = a*lam + c*h*acc + e1*x1 + e2*x2 on the owned cells (x1, x2 with their own
     // pending corrections)
-    if (x1) apply_pending<R>(a.pend[1], line, off, h, first, nc, cell, owned, p1v);
-    if (x2) apply_pending<R>(a.pend[2], line, off, h, first, nc, cell, owned, p2v);
+    if (hit1) {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (owned && cell[i] == q1.j) p1v[i] += q1.d;
+    }
+    if (hit2) {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (owned && cell[i] == q2.j) p2v[i] += q2.d;
+    }
     if (owned) {
         const double ch = a.c * h;
         double o[R];
@@ -505,6 +570,9 @@ void adjoint_stage_kernel(AdjArgs a)
         // lanes past the partials repeat the last one (the maximum is unchanged)
         apart = set[min(lane, a.anpart - 1) * a.apart_stride];
     }
+    // the corrections owed by lam, x1 and x2 (no x1, x2: none owed)
+    LineRec prec{0ull, 0.0};
+    if constexpr (ALPHAMAX) prec = pend_recs(a, line);
     const double *x1 = a.x1 ? a.x1 + off : nullptr;
     const double *x2 = a.x2 ? a.x2 + off : nullptr;
     const int first = ch.first, nc = ch.nc;
@@ -514,7 +582,7 @@ void adjoint_stage_kernel(AdjArgs a)
     double ab;
 #define HRW_ADJ_CHUNK(KK, IN)                                                                    \
     chunk_adjoint<KK, R, ALPHAMAX, IN>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,  \
-                                        ch.xmin, ch.rx, apart, line, off, first, nc,             \
+                                        ch.xmin, ch.rx, apart, prec, line, off, first, nc,       \
                                         &cand)
     if ((long long)W - R - (K - 1) >= 0 && (long long)W + 63LL * R + (K - 1) <= n) {
         if (K == 1) ab = HRW_ADJ_CHUNK(1, true);
@@ -533,13 +601,29 @@ void adjoint_stage_kernel(AdjArgs a)
             a.partial_out[wave] = wsum;
             s_part[threadIdx.x >> 6] = wsum;
             if (wmin != 0xffffffffu)
-                atomicMax(a.amax_out + line,
+                atomicMax(&a.rec_out[line].key,
                           ((unsigned long long)a.tag << 32) | (unsigned long long)(0xffffffffu - wmin));
         }
         __syncthreads();
         // used only for blocks whose four chunks lie on one line
         if (threadIdx.x == 0) a.bpartial_out[blockIdx.x] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
     }
+}
+
+// Record a pending correction per line (one wave per line), for a launch that reads the buffer
+// as x1 or x2 before any launch has read it as lam.
+static __global__ __launch_bounds__(64) void adjoint_resolve_kernel(Pending p, const long long *off,
+                                                                   const double *h,
+                                                                   const int *line_chunk0,
+                                                                   const int *line_nchunks,
+                                                                   int nlines)
+{
+    const int line = blockIdx.x;
+    if (line >= nlines) return;
+    const int j = pending_cell(p, line);
+    if (j < 0) return;
+    const double d = pending_delta<40>(p, j, off[line], h[line], line_chunk0[line], line_nchunks[line]);
+    if ((threadIdx.x & 63) == 0) p.rec[line].resolved = d;
 }
 
 // Apply a pending correction to a final output: one wave per line.
@@ -552,7 +636,8 @@ static __global__ __launch_bounds__(64) void adjoint_fixup_kernel(Pending p, dou
     if (line >= nlines) return;
     const int j = pending_cell(p, line);
     if (j < 0) return;
-    const double d = pending_delta(p, j, off[line], h[line], line_chunk0[line], line_nchunks[line]);
+    const double d = p.ready ? p.rec[line].resolved
+                             : pending_delta<40>(p, j, off[line], h[line], line_chunk0[line], line_nchunks[line]);
     if ((threadIdx.x & 63) == 0) out[off[line] + j] += d;
 }
 
