@@ -131,16 +131,22 @@ def run_plain(args, torch, hh, W, dev):
     if not bool(torch.isfinite(u).all()):
         raise RuntimeError("non-finite state after the timed steps")
     log(f"timed {args.steps} steps: {ms:.3f} ms")
-    fast_v2 = args.arith == "fast" and args.variant == 0
-    cpl = args.cells_per_lane or (6 if c.n >= (1 << 19) else 4)   # the library's automatic choice
-    kernel = ("hrw::stage1d_fast_kernel<3,LF,periodic,alphamax,%d,NPW> (NPW = 0 for stage 1, 1 for "
-              "stages 2-3)" % cpl
-              if fast_v2 else "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith)
+    # the library's automatic choice: lines of 2^19 cells or more run the persistent form with 6
+    # cells per lane, shorter ones one chunk per wavefront with 4
+    big = c.n >= (1 << 19)
+    cpl = args.cells_per_lane or (6 if big else 4)
+    persistent = args.variant == 2 or (args.variant == 0 and not args.cells_per_lane and big)
+    if args.arith != "fast" or args.variant == 1:
+        kernel = "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith
+    else:
+        kernel = ("hrw::%s<3,LF,periodic,alphamax,%d,NPW> (NPW = 0 for stage 1, 1 for stages 2-3)"
+                  % ("stage1d_pf_kernel" if persistent else "stage1d_fast_kernel", cpl))
     rl = roofline(64.0 / 3 * c.n, ms / (3 * args.steps), kernel, "stage1d_2p24")
     config = {"workload": "burgers1d_weno5_lf_rk3_periodic_2^24cells", "cells": c.n, "k": 3,
               "eps": 1e-6, "flux": "lax_friedrichs, alpha=max|u| per stage",
               "integrator": "rktvd order 3", "cfl": 0.4, "dt": c.dt, "arith": args.arith,
               "variant": args.variant, "cells_per_lane": cpl,
+              "form": "persistent" if persistent else "one chunk per wavefront",
               "parallelism": "single GPU"}
     return c.n * args.steps / (ms / 1e3), ms / args.steps, rl, config, \
         "synthetic (seeded SplitMix64: 8 sine modes + 16 jumps)"

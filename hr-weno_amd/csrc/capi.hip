@@ -711,18 +711,22 @@ static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
 {
     const bool amax0 = p->alphamax();
-    if (p->arith == HRW_ARITH_FAST && (p->variant == 0 || p->variant == 2) &&
+    if (p->arith == HRW_ARITH_FAST && p->variant != 1 &&
         a.n < (1LL << 31) - (1LL << 20) &&
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
         const bool per = p->bc == HRW_BC_PERIODIC;
-        // cells per lane: tuned, or 6 (measured best on 2^20..2^24 cells), 4 on short lines
-        // (more wavefronts to hide the load latency)
-        const int cpl = p->cells_per_lane ? p->cells_per_lane : (a.n >= (1LL << 19) ? 6 : 4);
-        const int r = p->variant == 2 ? -cpl : cpl;   // < 0: persistent
-        if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, r, a, s);
-        else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, r, a, s);
-        else launch_fast_1d_k3(p->numflux, per, amax0, r, a, s);
+        // automatic (variant 0, cells_per_lane 0): on lines of 2^19 cells or more the
+        // persistent form with 6 cells per lane, on shorter ones one chunk per wavefront with 4
+        // (more wavefronts to hide the load latency) -- measured best on 2^16..2^24 cells.
+        // The persistent form computes the widths from the linear grid: no width array.
+        const bool big = a.n >= (1LL << 19);
+        const int cpl = p->cells_per_lane ? p->cells_per_lane : (big ? 6 : 4);
+        const bool persistent = p->variant == 2 || (p->variant == 0 && p->cells_per_lane == 0 && big);
+        const int form = persistent && !a.width ? FORM_PERSISTENT : FORM_CHUNK;
+        if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, form, cpl, a, s);
+        else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, form, cpl, a, s);
+        else launch_fast_1d_k3(p->numflux, per, amax0, form, cpl, a, s);
         return;
     }
     const bool per = p->bc == HRW_BC_PERIODIC, fast = p->arith == HRW_ARITH_FAST;

@@ -220,8 +220,11 @@ __device__ __forceinline__ void interior_range(long long n, int R, int K, long l
 
 // INTERIOR: the chunk is interior and d holds its loads (issued by the caller, so that the
 // stage itself issues no load); otherwise d is not read and the cells are loaded here with
-// the boundary handling.
-template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR>
+// the boundary handling.  LINEAR: the line has no width array (the widths come from the linear
+// grid), so the stage has no load at all on an interior chunk -- the persistent forms need
+// that: a load on only some paths makes the compiler's wait after the paths join cover every
+// load in flight, the next chunk's prefetch included.
+template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR, bool LINEAR = false>
 __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha,
                                               const ChunkData<K, R, NPW> &d)
 {
@@ -281,7 +284,8 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     // ---- width of cell g (recomputed from the edges like the reference, or the given array)
     auto edge = [&](int g) { return linear_edge32(a.xmin, a.rx, g < 0 ? 0 : (g < n ? g : n)); };
     auto width_of = [&](int g, double e_lo, double e_hi) {
-        return a.width ? a.width[g < 0 ? 0 : (g < n ? g : n - 1)] : e_hi - e_lo;
+        if constexpr (LINEAR) return e_hi - e_lo;
+        else return a.width ? a.width[g < 0 ? 0 : (g < n ? g : n - 1)] : e_hi - e_lo;
     };
 
     // ---- stage combination and store of cell t (global g); un/L of a pair's first cell wait
@@ -452,7 +456,7 @@ __device__ __forceinline__ double alpha_reduce(const Stage1DArgs &a, unsigned lo
     return alpha;
 }
 
-// Persistent form: wave w of the grid takes the interior chunks lo + w, lo + w + N, ... (N =
+// Persistent form (lines without a width array): wave w of the grid takes the interior chunks lo + w, lo + w + N, ... (N =
 // waves in the grid) and loads the next one into a second register set before it computes the
 // current one, so every wave keeps a chunk of loads in flight behind its FP64 work.  The loads
 // are unconditional (the last iteration reloads its own chunk, an L2 hit): a load on only some
@@ -477,7 +481,7 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
         for (; c <= hi; c += nw) {
             const long long c1 = c + nw <= hi ? c + nw : c;
             load_chunk<K, R, NPW>(a, (int)(c1 * cw), nxt);
-            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, (int)(c * cw), alpha, cur));
+            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true, true>(a, (int)(c * cw), alpha, cur));
             cur = nxt;
         }
     } else {
@@ -488,7 +492,7 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
     const long long nb = lo + (nch - 1 - hh);
     for (long long b = w; b < nb; b += nw) {
         const long long c = b < lo ? b : hh + 1 + (b - lo);
-        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, (int)(c * cw), alpha, cur));
+        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false, true>(a, (int)(c * cw), alpha, cur));
     }
     if constexpr (ALPHAMAX) publish_max(m, a.alpha_out, s_red);
 }

@@ -49,11 +49,11 @@ static void l1p(const Stage1DArgs &a, hipStream_t s)
     }
 }
 template <int K, int FLUX, bool PER, bool AMAX>
-static void l1f_r(int r, const Stage1DArgs &a, hipStream_t s)
+static void l1f_r(int form, int r, const Stage1DArgs &a, hipStream_t s)
 {
-    if (r < 0) {
-        if (r == -4) l1p<K, FLUX, PER, AMAX, 4>(a, s);
-        else if (r == -6) l1p<K, FLUX, PER, AMAX, 6>(a, s);
+    if (form != FORM_CHUNK) {
+        if (r == 4) l1p<K, FLUX, PER, AMAX, 4>(a, s);
+        else if (r == 6) l1p<K, FLUX, PER, AMAX, 6>(a, s);
         else l1p<K, FLUX, PER, AMAX, 8>(a, s);
         return;
     }
@@ -62,22 +62,22 @@ static void l1f_r(int r, const Stage1DArgs &a, hipStream_t s)
     else l1f<K, FLUX, PER, AMAX, 8>(a, s);
 }
 template <int K, int FLUX, bool PER>
-static void l1f_a(bool amax, int r, const Stage1DArgs &a, hipStream_t s)
+static void l1f_a(bool amax, int form, int r, const Stage1DArgs &a, hipStream_t s)
 {
     if constexpr (FLUX == LF) {
-        if (amax) return l1f_r<K, FLUX, PER, true>(r, a, s);
+        if (amax) return l1f_r<K, FLUX, PER, true>(form, r, a, s);
     }
-    l1f_r<K, FLUX, PER, false>(r, a, s);
+    l1f_r<K, FLUX, PER, false>(form, r, a, s);
 }
 template <int K>
-void l1f_x(int flux, bool per, bool amax, int r, const Stage1DArgs &a, hipStream_t s)
+void l1f_x(int flux, bool per, bool amax, int form, int r, const Stage1DArgs &a, hipStream_t s)
 {
     if (flux == LF) {
-        if (per) l1f_a<K, LF, true>(amax, r, a, s);
-        else l1f_a<K, LF, false>(amax, r, a, s);
+        if (per) l1f_a<K, LF, true>(amax, form, r, a, s);
+        else l1f_a<K, LF, false>(amax, form, r, a, s);
     } else {
-        if (per) l1f_a<K, GODUNOV, true>(amax, r, a, s);
-        else l1f_a<K, GODUNOV, false>(amax, r, a, s);
+        if (per) l1f_a<K, GODUNOV, true>(amax, form, r, a, s);
+        else l1f_a<K, GODUNOV, false>(amax, form, r, a, s);
     }
 }
 
@@ -86,10 +86,10 @@ void l1f_x(int flux, bool per, bool amax, int r, const Stage1DArgs &a, hipStream
 
 #define HRW_CAT2(a, b) a##b
 #define HRW_CAT(a, b) HRW_CAT2(a, b)
-void HRW_CAT(launch_fast_1d_k, HRW_K)(int flux, bool periodic, bool alphamax, int r, const Stage1DArgs &a,
-                                      hipStream_t s)
+void HRW_CAT(launch_fast_1d_k, HRW_K)(int flux, bool periodic, bool alphamax, int form, int r,
+                                      const Stage1DArgs &a, hipStream_t s)
 {
-    l1f_x<HRW_K>(flux, periodic, alphamax, r, a, s);
+    l1f_x<HRW_K>(flux, periodic, alphamax, form, r, a, s);
 }
 
 }  // namespace hrw

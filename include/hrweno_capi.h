@@ -153,11 +153,13 @@ int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, d
                       double x2min, double x2max, double g1a, double g1b, double g2a,
                       double g2b, int arith, hrw_problem_t *out);
 /* Kernel choice for the 1D FAST form (no reference counterpart; used by the autotuner):
- * variant 0 = register-blocked kernel with cells_per_lane (4, 6 or 8; 0 = chosen by the line
- * length, the default) consecutive cells per lane,
+ * variant 0 = register-blocked kernel with cells_per_lane (4, 6 or 8) consecutive cells per
+ * lane, one chunk per wavefront; with cells_per_lane 0 (the default) the form and the cells per
+ * lane are chosen by the line length (lines of 2^19 cells or more: variant 2 with 6),
  * variant 1 = LDS-tile kernel, variant 2 = the register-blocked kernel run persistently with
- * the next chunk's loads issued ahead of the current chunk's arithmetic.  Variants 0 and 2
- * give identical results; variant 1 agrees within the FAST tolerance.  For a 2D problem
+ * the next chunk's loads issued ahead of the current chunk's arithmetic (lines with a width
+ * array take the variant 0 kernel).  Variants 0 and 2 give identical results; variant 1
+ * agrees within the FAST tolerance.  For a 2D problem
  * variant must be 0 and cells_per_lane is the number of rows each wavefront marches over
  * (0 = chosen at launch so that every wavefront is resident at once); results are identical
  * for every value. */

@@ -77,7 +77,8 @@ def test_tiling_invariance_exact(oracle, hh, dev, n):
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_fast_kernels_all_sizes(oracle, hh, dev, k, bc):
     """FAST form through the 1D kernels (register-blocked with 4, 6 or 8 cells per lane or
-    chosen per stage, its persistent prefetching form, and the LDS tile) on line lengths that
+    chosen by the line length, its persistent prefetching form, and the LDS tile) on line
+    lengths that
     hit every edge/interior case (chunks of 62R output cells with R + K - 1 halo cells before
     and 63R + K - 1 after): each within the FAST tolerance of the oracle, and the
     register-blocked forms bit-identical (the per-cell arithmetic does not depend on which
