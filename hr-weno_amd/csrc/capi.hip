@@ -716,14 +716,16 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
         const bool per = p->bc == HRW_BC_PERIODIC;
-        // automatic (variant 0, cells_per_lane 0): on lines of 2^19 cells or more the
-        // persistent form with 6 cells per lane, on shorter ones one chunk per wavefront with 4
-        // (more wavefronts to hide the load latency) -- measured best on 2^16..2^24 cells.
-        // The persistent form computes the widths from the linear grid: no width array.
-        const bool big = a.n >= (1LL << 19);
-        const int cpl = p->cells_per_lane ? p->cells_per_lane : (big ? 6 : 4);
-        const bool persistent = p->variant == 2 || (p->variant == 0 && p->cells_per_lane == 0 && big);
-        const int form = persistent && !a.width ? FORM_PERSISTENT : FORM_CHUNK;
+        // automatic (variant 0, cells_per_lane 0), measured best on 2^16..2^24 cells: lines of
+        // 2^21 cells or more (several chunks per resident wavefront) the persistent form with 4
+        // cells per lane; 2^19..2^21 one chunk per wavefront with 6; shorter lines one chunk per
+        // wavefront with 4 (more wavefronts to hide the load latency).  The persistent form
+        // computes the widths from the linear grid: lines with a width array take the other.
+        const bool automatic = p->variant == 0 && p->cells_per_lane == 0;
+        const bool persistent = (p->variant == 2 || (automatic && a.n >= (1LL << 21))) && !a.width;
+        const int cpl = p->cells_per_lane ? p->cells_per_lane
+                                          : (!persistent && a.n >= (1LL << 19) ? 6 : 4);
+        const int form = persistent ? FORM_PERSISTENT : FORM_CHUNK;
         if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, form, cpl, a, s);
         else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, form, cpl, a, s);
         else launch_fast_1d_k3(p->numflux, per, amax0, form, cpl, a, s);
