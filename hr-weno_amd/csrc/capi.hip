@@ -574,6 +574,7 @@ struct hrw_problem_s {
     // 1D
     double xmin = 0, rx = 0;
     double *d_width = nullptr;
+    double *d_cnu = nullptr;          // non-uniform grid: (n, k+1, k)
     // 2D
     int n1 = 0, n2 = 0, ty = 0;      // rows per wavefront, 0 = chosen at launch
     double x1min = 0, rx1 = 0, x2min = 0, rx2 = 0, g1a = 0, g1b = 0, g2a = 0, g2b = 0;
@@ -586,9 +587,35 @@ struct hrw_problem_s {
     ~hrw_problem_s()
     {
         dfree(d_width);
+        dfree(d_cnu);
         dfree(slots);
     }
 };
+
+extern "C" int hrw_problem_burgers1d_edges(int n, int k, double eps, int numflux, int bc,
+                                           double alpha, const double *xedges, int arith,
+                                           hrw_problem_t *out)
+{
+    if (!out) return fail(HRW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!xedges) return fail(HRW_ERR_ARG, "null xedges");
+    if (bc != HRW_BC_ZERO_GRADIENT)
+        return fail(HRW_ERR_ARG, "a non-uniform grid (xedges) needs HRW_BC_ZERO_GRADIENT");
+    if (!(n > 0)) return fail(HRW_ERR_ARG, "Invalid input 'ncells'. Valid range: ncells > 0.");
+    std::vector<double> width((size_t)n);
+    for (int i = 0; i < n; ++i) width[(size_t)i] = xedges[i + 1] - xedges[i];   // src/hrweno_grids.f90:247
+    hrw_problem_t p = nullptr;
+    CALL(hrw_problem_burgers1d(n, k, eps, numflux, bc, alpha, xedges[0], xedges[n], width.data(),
+                               arith, &p));
+    int rc = dalloc(&p->d_cnu, (size_t)n * k * (k + 1));
+    if (rc == HRW_OK) rc = compute_cnu(n, k, xedges, p->d_cnu);
+    if (rc != HRW_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return HRW_OK;
+}
 
 extern "C" int hrw_problem_burgers1d(int n, int k, double eps, int numflux, int bc, double alpha,
                                      double xmin, double xmax, const double *width, int arith,
@@ -711,7 +738,7 @@ static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
 {
     const bool amax0 = p->alphamax();
-    if (p->arith == HRW_ARITH_FAST && p->variant != 1 &&
+    if (p->arith == HRW_ARITH_FAST && p->variant != 1 && !a.cnu &&
         a.n < (1LL << 31) - (1LL << 20) &&
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
@@ -774,6 +801,7 @@ static int run_stage_sets(const hrw_problem_s *p, const unsigned long long *ain,
         a.Lo4 = io.Lo4;
         a.Lout = io.Lout;
         a.width = p->d_width;
+        a.cnu = p->d_cnu;
         a.xmin = p->xmin;
         a.rx = p->rx;
         a.n = p->neq;

@@ -159,6 +159,8 @@ struct Stage1DArgs {
     const double *Lo4;   // MS3: L(u_{n-4})     (may alias Lout)
     double *Lout;        // L(us) (MS3 history / rhs-only), or nullptr
     const double *width; // cell widths, or nullptr for the linear-grid formula
+    const double *cnu;   // non-uniform grid: cnu(j, r, i) of weno_calc_cnu, (n, K+1, K), or
+                         // nullptr for the uniform coefficients
     double xmin, rx;     // linear grid: edge i = xmin + rx*i
     long long n;
     double eps, h, alpha;
@@ -202,8 +204,18 @@ __global__ __launch_bounds__(kBlock) void stage1d_kernel(Stage1DArgs a)
         const int c = tid + m * kBlock;
         const double *x = s_u + c;   // window centre at s_u[c+K-1]
         double vl, vr;
-        if constexpr (FAST) weno_cell_fast<K>(x, a.eps, vl, vr);
-        else weno_cell_exact<K>(x, a.eps, nullptr, vl, vr);
+        if (a.cnu) {
+            // non-uniform grid (src/hrweno_weno.f90:174-178): the cell's own coefficients, in
+            // the reference's order in both arithmetic forms; cells outside the line, whose
+            // faces the boundary fluxes replace, take the nearest cell's
+            const long long g = base - 1 + c;
+            const long long gi = g < 0 ? 0 : (g >= n ? n - 1 : g);
+            weno_cell_exact<K>(x, a.eps, a.cnu + gi * (K * (K + 1)), vl, vr);
+        } else if constexpr (FAST) {
+            weno_cell_fast<K>(x, a.eps, vl, vr);
+        } else {
+            weno_cell_exact<K>(x, a.eps, nullptr, vl, vr);
+        }
         s_vl[c] = vl;
         s_vr[c] = vr;
     }

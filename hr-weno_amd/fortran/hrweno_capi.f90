@@ -141,6 +141,15 @@ module hrweno_capi
          type(c_ptr), intent(out) :: out
       end function
 
+      integer(c_int) function hrw_problem_burgers1d_edges(n, k, eps, numflux, bc, alpha, &
+                                                          xedges, arith, out) bind(C)
+         import :: c_int, c_double, c_ptr
+         integer(c_int), value :: n, k, numflux, bc, arith
+         real(c_double), value :: eps, alpha
+         type(c_ptr), value :: xedges
+         type(c_ptr), intent(out) :: out
+      end function
+
       integer(c_int) function hrw_problem_pbe2d(n1, n2, k, eps, bc, x1min, x1max, x2min, &
                                                 x2max, g1a, g1b, g2a, g2b, arith, out) bind(C)
          import :: c_int, c_double, c_ptr

@@ -26,14 +26,16 @@ module hrweno_problems
 
 contains
 
-   function burgers1d(ncells, k, eps, flux, bc, alpha, xmin, xmax, width, arith) result(p)
+   function burgers1d(ncells, k, eps, flux, bc, alpha, xmin, xmax, width, arith, xedges) result(p)
    !! 1D Burgers u_t + (u**2/2)_x = 0 on a linear grid [xmin, xmax] (or given cell widths).
    !! flux: HRW_LAX_FRIEDRICHS (alpha < 0: max|v| of each stage input) or
    !! HRW_GODUNOV; bc: HRW_BC_PERIODIC or HRW_BC_ZERO_GRADIENT (example 1).
+   !! xedges(0:ncells): a non-uniform grid whose own WENO coefficients the reconstruction uses,
+   !! as weno(ncells, k, eps, xedges) (src/hrweno_weno.f90:100-112); needs HRW_BC_ZERO_GRADIENT.
       integer, intent(in) :: ncells
       integer, intent(in), optional :: k, flux, bc, arith
       real(rk), intent(in), optional :: eps, alpha, xmin, xmax
-      real(rk), intent(in), optional, target :: width(:)
+      real(rk), intent(in), optional, target :: width(:), xedges(:)
       type(problem) :: p
       integer :: k_, flux_, bc_, arith_
       real(rk) :: eps_, alpha_, xmin_, xmax_
@@ -48,6 +50,14 @@ contains
       if (present(alpha)) alpha_ = alpha
       if (present(xmin)) xmin_ = xmin
       if (present(xmax)) xmax_ = xmax
+      if (present(xedges)) then
+         if (size(xedges) /= ncells + 1) error stop "Invalid input 'xedges': size(xedges) /= ncells + 1."
+         if (.not. is_contiguous(xedges)) error stop "Invalid input 'xedges': not contiguous."
+         if (hrw_problem_burgers1d_edges(ncells, k_, eps_, flux_, bc_, alpha_, c_loc(xedges), &
+                                         arith_, p%h) /= 0) error stop hrw_message()
+         p%neq = ncells
+         return
+      end if
       pw = c_null_ptr
       if (present(width)) then
          if (size(width) /= ncells) error stop "Invalid input 'width': size(width) /= ncells."

@@ -56,6 +56,7 @@ SIGNATURES = {
     "hrw_grid_geometric": [D, D, D, I, PD],
     "hrw_grid_compute": [I, PD, PD, PD],
     "hrw_problem_burgers1d": [I, I, D, I, I, D, D, D, PD, I, C.POINTER(P)],
+    "hrw_problem_burgers1d_edges": [I, I, D, I, I, D, PD, I, C.POINTER(P)],
     "hrw_problem_pbe2d": [I, I, I, D, I, D, D, D, D, D, D, D, D, I, C.POINTER(P)],
     "hrw_problem_tune": [P, I, I],
     "hrw_problem_destroy": [P],

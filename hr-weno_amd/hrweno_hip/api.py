@@ -248,15 +248,24 @@ class Problem:
 
 
 class Burgers1D(Problem):
-    """1D Burgers u_t + (u**2/2)_x = 0 (example/example1_burgers_1d_fv.f90:72-122)."""
+    """1D Burgers u_t + (u**2/2)_x = 0 (example/example1_burgers_1d_fv.f90:72-122).  With
+    xedges (n+1 edges, zero-gradient boundaries) the reconstruction uses the non-uniform grid's
+    own coefficients, as weno(ncells, k, eps, xedges) (src/hrweno_weno.f90:100-112)."""
 
     def __init__(self, n, k=3, eps=1e-6, flux=LAX_FRIEDRICHS, bc=BC_PERIODIC, alpha=-1.0,
-                 xmin=-1.0, xmax=1.0, width=None, arith="fast"):
+                 xmin=-1.0, xmax=1.0, width=None, arith="fast", xedges=None):
         h = C.c_void_p()
-        w = _np(width) if width is not None else None
-        call("hrw_problem_burgers1d", int(n), int(k), float(eps), int(flux), int(bc),
-             float(alpha), float(xmin), float(xmax), _pd(w) if w is not None else None,
-             _ARITH[arith], C.byref(h))
+        if xedges is not None:
+            xe = _np(xedges)
+            if xe.size != n + 1:
+                raise HrwError(1, "Invalid input 'xedges': size(xedges) /= ncells + 1.")
+            call("hrw_problem_burgers1d_edges", int(n), int(k), float(eps), int(flux), int(bc),
+                 float(alpha), _pd(xe), _ARITH[arith], C.byref(h))
+        else:
+            w = _np(width) if width is not None else None
+            call("hrw_problem_burgers1d", int(n), int(k), float(eps), int(flux), int(bc),
+                 float(alpha), float(xmin), float(xmax), _pd(w) if w is not None else None,
+                 _ARITH[arith], C.byref(h))
         self._h = h
 
 

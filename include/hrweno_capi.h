@@ -146,6 +146,14 @@ int hrw_grid_compute(int ncells, const double *edges, double *center, double *wi
 int hrw_problem_burgers1d(int n, int k, double eps, int numflux, int bc, double alpha,
                           double xmin, double xmax, const double *width, int arith,
                           hrw_problem_t *out);
+/* The same on the non-uniform grid of edges xedges (host, n+1): the example-1 right-hand side
+ * with weno(ncells, k, eps, xedges) -- reconstruction with the grid's own coefficients cnu
+ * (src/hrweno_weno.f90:100-112,174-178,221-297) and widths xedges(i+1)-xedges(i)
+ * (src/hrweno_grids.f90:247).  bc must be HRW_BC_ZERO_GRADIENT (the reference extends the
+ * edges linearly and the cell values by a constant beyond both ends).  The reconstruction
+ * takes the reference's evaluation order in both arithmetic forms. */
+int hrw_problem_burgers1d_edges(int n, int k, double eps, int numflux, int bc, double alpha,
+                                const double *xedges, int arith, hrw_problem_t *out);
 /* 2D population balance n_t + (g1 n)_x1 + (g2 n)_x2 = 0, g1 = g1a + g1b*x1,
  * g2 = g2a + g2b*x2, Godunov flux, linear grids, u((j-1)*n1 + i) --
  * example/example2_pbe_2d_fv.f90:73-155.  bc: HRW_BC_ZERO_FLUX or HRW_BC_INFLOW0_OUTFLOW. */

@@ -502,11 +502,12 @@ int64_t hro_burgers_rk(int n, int k, double eps, int flux, int bc, double alpha_
 }
 
 int hro_burgers_ms(int n, int k, double eps, int flux, int bc, double alpha_fixed,
-                   const double *width, double *u, double *t, double tout, double dt)
+                   const double *width, const double *cnu, double *u, double *t, double tout,
+                   double dt)
 {
     hro_burgers_ctx c;
     c.n = n; c.k = k; c.flux = flux; c.bc = bc; c.eps = eps; c.alpha_fixed = alpha_fixed;
-    c.width = width; c.cnu = NULL;
+    c.width = width; c.cnu = cnu;
     c.work = (double *)malloc((size_t)5 * (n + 2 * k + 2) * sizeof(double));
     if (!c.work) return -2;
     hro_ode ode;

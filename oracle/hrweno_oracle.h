@@ -116,7 +116,8 @@ int64_t hro_burgers_rk(int n, int k, double eps, int flux, int bc, double alpha_
                        int *fevals);
 /* Burgers with mstvd from istate=1; returns fevals. */
 int hro_burgers_ms(int n, int k, double eps, int flux, int bc, double alpha_fixed,
-                   const double *width, double *u, double *t, double tout, double dt);
+                   const double *width, const double *cnu, double *u, double *t, double tout,
+                   double dt);
 /* 2D PBE from istate=1 with mstvd (order 0) or rktvd (order 1..3); returns fevals. */
 int hro_pbe_ms(int n1, int n2, int k, double eps, int bc, double g1a, double g1b,
                double g2a, double g2b, const double *edges1, const double *edges2,

@@ -68,7 +68,7 @@ class Oracle:
         lib.hro_burgers_rk.restype = _L
         lib.hro_pbe_ms.argtypes = [_I, _I, _I, _D, _I, _D, _D, _D, _D, _P, _P, _I, _P, _P, _D,
                                    _D]
-        lib.hro_burgers_ms.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P, _D, _D]
+        lib.hro_burgers_ms.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P, _P, _D, _D]
         lib.hro_linear_ode.argtypes = [_I, _P, _I, _P, _P, _D, _D]
         lib.hro_burgers_rhs.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P]
         lib.hro_burgers_rhs_vjp.argtypes = [_I, _I, _D, _D, _P, _P, _P, _P]
@@ -213,12 +213,14 @@ class Oracle:
         return u, t.value, fe
 
     def burgers_ms(self, u0, width, *, k=3, eps=1e-6, flux=FLUX_LF, bc=BC_PERIODIC, alpha=-1.0,
-                   t0=0.0, tout=1.0, dt=1e-3):
+                   t0=0.0, tout=1.0, dt=1e-3, cnu=None):
         """Advance 1D Burgers with mstvd from a fresh object; returns (u, t, fevals)."""
         u = _f64(u0).copy()
         w = _f64(width)
+        cn = _f64(cnu) if cnu is not None else None
         t = C.c_double(t0)
-        fe = self.lib.hro_burgers_ms(u.size, k, eps, flux, bc, alpha, _ptr(w), _ptr(u),
+        fe = self.lib.hro_burgers_ms(u.size, k, eps, flux, bc, alpha, _ptr(w),
+                                     _ptr(cn) if cn is not None else None, _ptr(u),
                                      C.byref(t), tout, dt)
         return u, t.value, fe
 

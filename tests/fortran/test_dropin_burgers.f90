@@ -70,6 +70,25 @@ program test_dropin_burgers
    call check("ms t", t_fused, t_cb, thr=0.0_rk)
    call check_int("ms fevals", ms_fused%fevals, ms_cb%fevals)
    call check_array("ms callback == fused", u_fused, u_cb, thr=0.0_rk)
+
+   ! non-uniform grid: weno(..., xedges) in the callback equals burgers1d(..., xedges=)
+   call gx%geometric(xmin=1.0_rk, xmax=11.0_rk, ratio=1.02_rk, ncells=nc)
+   myweno = weno(ncells=nc, k=3, eps=1e-6_rk, xedges=gx%edges, arith=HRW_ARITH_EXACT)
+   call prob%destroy()
+   prob = burgers1d(nc, k=3, eps=1e-6_rk, flux=HRW_GODUNOV, bc=HRW_BC_ZERO_GRADIENT, &
+                    xedges=gx%edges, arith=HRW_ARITH_EXACT)
+   ode_cb = rktvd(rhs, nc, order=3)
+   ode_fused = rktvd(prob, 3)
+   do i = 1, nc
+      u_cb(i) = ic(gx%center(i) - 6.0_rk)
+   end do
+   u_fused = u_cb
+   t_cb = 0.0_rk
+   t_fused = 0.0_rk
+   call ode_cb%integrate(u_cb, t_cb, 2.0_rk, dt)
+   call ode_fused%integrate(u_fused, t_fused, 2.0_rk, dt)
+   call check("nonuniform t", t_fused, t_cb, thr=0.0_rk)
+   call check_array("nonuniform callback == fused", u_fused, u_cb, thr=0.0_rk)
    call finish("dropin_burgers")
 
 contains

@@ -95,3 +95,12 @@ def test_weno_tables_match_oracle(hh, oracle):
         assert np.array_equal(d, od) and np.array_equal(c.T, oc)
     with pytest.raises(hh.HrwError, match="Invalid input 'k'. Valid range: 1 <= k <= 3."):
         hh.tables(4)
+
+
+def test_nonuniform_problem_argument_errors(hh):
+    """The non-uniform-grid problem checks its arguments before touching the device."""
+    xe = np.linspace(-1.0, 1.0, 11)
+    with pytest.raises(hh.HrwError, match="needs HRW_BC_ZERO_GRADIENT"):
+        hh.Burgers1D(10, bc=hh.BC_PERIODIC, xedges=xe)
+    with pytest.raises(hh.HrwError, match=r"size\(xedges\) /= ncells \+ 1"):
+        hh.Burgers1D(12, bc=hh.BC_ZERO_GRADIENT, xedges=xe)
