@@ -738,7 +738,13 @@ static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
 {
     const bool amax0 = p->alphamax();
-    if (p->arith == HRW_ARITH_FAST && p->variant != 1 && !a.cnu &&
+    // the register-blocked kernels take linear grids whose edges stay within 1e8 cell widths
+    // of the origin (their 1/width is one Newton step from 1/rx); a width array, a non-uniform
+    // grid's coefficients or a grid far from the origin go to the tile kernel
+    const bool linear_near = !a.cnu && !a.width &&
+                             std::max(std::fabs(a.xmin), std::fabs(a.xmin + a.rx * (double)a.n)) <=
+                                 1e8 * a.rx;
+    if (p->arith == HRW_ARITH_FAST && p->variant != 1 && linear_near &&
         a.n < (1LL << 31) - (1LL << 20) &&
         aligned16(a.us) && aligned16(a.u0) && aligned16(a.uo) && aligned16(a.uo4) &&
         aligned16(a.Lo4) && aligned16(a.Lout)) {
@@ -746,10 +752,9 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
         // automatic (variant 0, cells_per_lane 0), measured best on 2^16..2^24 cells: lines of
         // 2^21 cells or more (several chunks per resident wavefront) the persistent form with 4
         // cells per lane; 2^19..2^21 one chunk per wavefront with 6; shorter lines one chunk per
-        // wavefront with 4 (more wavefronts to hide the load latency).  The persistent form
-        // computes the widths from the linear grid: lines with a width array take the other.
+        // wavefront with 4 (more wavefronts to hide the load latency).
         const bool automatic = p->variant == 0 && p->cells_per_lane == 0;
-        const bool persistent = (p->variant == 2 || (automatic && a.n >= (1LL << 21))) && !a.width;
+        const bool persistent = p->variant == 2 || (automatic && a.n >= (1LL << 21));
         const int cpl = p->cells_per_lane ? p->cells_per_lane
                                           : (!persistent && a.n >= (1LL << 19) ? 6 : 4);
         const int form = persistent ? FORM_PERSISTENT : FORM_CHUNK;
@@ -1230,6 +1235,10 @@ extern "C" int hrw_batch_burgers_create(int64_t nlines, const int *ncells, const
         if (!(dt[l] != 0.0) || !std::isfinite(dt[l])) return fail(HRW_ERR_ARG, "dt must be finite and nonzero");
         const double x0 = xmin ? xmin[l] : -1.0, x1 = xmax ? xmax[l] : 1.0;
         if (!(x1 > x0)) return fail(HRW_ERR_ARG, "Invalid input 'xmin', 'xmax'. Valid range: xmax > xmin.");
+        // the stage kernel's 1/width starts from 1/rx (accurate while the edges stay within
+        // 1e8 cell widths of the origin), the same as the single-line path's
+        if (!(std::max(std::fabs(x0), std::fabs(x1)) <= 1e8 * ((x1 - x0) / ncells[l])))
+            return fail(HRW_ERR_ARG, "line %lld: the batched path needs max(|xmin|, |xmax|) <= 1e8 cell widths", (long long)l);
         b->h_off[l] = off;
         off += ((int64_t)ncells[l] + 31) / 32 * 32;   // 256-byte aligned line starts
         b->h_n.push_back(ncells[l]);

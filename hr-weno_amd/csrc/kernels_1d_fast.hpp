@@ -220,11 +220,14 @@ __device__ __forceinline__ void interior_range(long long n, int R, int K, long l
 
 // INTERIOR: the chunk is interior and d holds its loads (issued by the caller, so that the
 // stage itself issues no load); otherwise d is not read and the cells are loaded here with
-// the boundary handling.  LINEAR: the line has no width array (the widths come from the linear
-// grid), so the stage has no load at all on an interior chunk -- the persistent forms need
-// that: a load on only some paths makes the compiler's wait after the paths join cover every
-// load in flight, the next chunk's prefetch included.
-template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR, bool LINEAR = false>
+// the boundary handling.  The line is on a linear grid (lines with a width array take the tile
+// kernel), so an interior chunk's stage has no load at all -- the persistent form needs that: a
+// load on only some paths makes the compiler's wait after the paths join cover every load in
+// flight, the next chunk's prefetch included.  NEWTON_W: 1/width by one Newton step from
+// 1/rx (the host keeps the line's edges within 1e8 cell widths of the origin, where every
+// width is rx to ~1e-8 and one step gives 1/w to a few ulp); otherwise by the hardware estimate
+// with two steps.
+template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR, bool NEWTON_W = true>
 __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha,
                                               const ChunkData<K, R, NPW> &d)
 {
@@ -281,11 +284,16 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     // a cell is real (inside the line) -- every cell of an interior chunk is
     auto real = [&](int g) { return INTERIOR || (g >= 0 && g < n); };
 
-    // ---- width of cell g (recomputed from the edges like the reference, or the given array)
-    auto edge = [&](int g) { return linear_edge32(a.xmin, a.rx, g < 0 ? 0 : (g < n ? g : n)); };
-    auto width_of = [&](int g, double e_lo, double e_hi) {
-        if constexpr (LINEAR) return e_hi - e_lo;
-        else return a.width ? a.width[g < 0 ? 0 : (g < n ? g : n - 1)] : e_hi - e_lo;
+    // ---- width of cell g, recomputed from the edges like the reference
+    // (src/hrweno_grids.f90:76-78,247), and its reciprocal (see NEWTON_W)
+    auto edge = [&](int g) {
+        return linear_edge32(a.xmin, a.rx, INTERIOR ? g : (g < 0 ? 0 : (g < n ? g : n)));
+    };
+    auto width_of = [&](int, double e_lo, double e_hi) { return e_hi - e_lo; };
+    [[maybe_unused]] const double inv_rx = NEWTON_W ? rcp_nr(a.rx) : 0.0;
+    auto inv_width = [&](double w) {
+        if constexpr (NEWTON_W) return fma_(inv_rx, fma_(-w, inv_rx, 1.0), inv_rx);
+        else return rcp_nr(w);
     };
 
     // ---- stage combination and store of cell t (global g); un/L of a pair's first cell wait
@@ -298,7 +306,7 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
                 if (g == n - 1) fright = fleft;
             }
         }
-        L = (fleft - fright) * rcp_nr(w);
+        L = (fleft - fright) * inv_width(w);
         const double us = v[H + t];
         double tt = sc.A * us;
         if constexpr (NPW >= 1) tt = fma_(sc.B, p1[t], tt);
@@ -407,6 +415,8 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     return owned ? m_out : 0.0;
 }
 
+// One chunk with its own loads (the batch kernel; the host checks its lines against the
+// NEWTON_W bound, so that a line gives the single-line kernels' bits).
 template <int K, int FLUX, bool PERIODIC, int R, int NPW>
 __device__ __forceinline__ double chunk_load_stage(const Stage1DArgs &a, int W, double alpha)
 {
@@ -456,7 +466,7 @@ __device__ __forceinline__ double alpha_reduce(const Stage1DArgs &a, unsigned lo
     return alpha;
 }
 
-// Persistent form (lines without a width array): wave w of the grid takes the interior chunks lo + w, lo + w + N, ... (N =
+// Persistent form: wave w of the grid takes the interior chunks lo + w, lo + w + N, ... (N =
 // waves in the grid) and loads the next one into a second register set before it computes the
 // current one, so every wave keeps a chunk of loads in flight behind its FP64 work.  The loads
 // are unconditional (the last iteration reloads its own chunk, an L2 hit): a load on only some
@@ -481,7 +491,7 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
         for (; c <= hi; c += nw) {
             const long long c1 = c + nw <= hi ? c + nw : c;
             load_chunk<K, R, NPW>(a, (int)(c1 * cw), nxt);
-            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true, true>(a, (int)(c * cw), alpha, cur));
+            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, (int)(c * cw), alpha, cur));
             cur = nxt;
         }
     } else {
@@ -492,7 +502,7 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
     const long long nb = lo + (nch - 1 - hh);
     for (long long b = w; b < nb; b += nw) {
         const long long c = b < lo ? b : hh + 1 + (b - lo);
-        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false, true>(a, (int)(c * cw), alpha, cur));
+        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, (int)(c * cw), alpha, cur));
     }
     if constexpr (ALPHAMAX) publish_max(m, a.alpha_out, s_red);
 }

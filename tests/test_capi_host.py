@@ -104,3 +104,10 @@ def test_nonuniform_problem_argument_errors(hh):
         hh.Burgers1D(10, bc=hh.BC_PERIODIC, xedges=xe)
     with pytest.raises(hh.HrwError, match=r"size\(xedges\) /= ncells \+ 1"):
         hh.Burgers1D(12, bc=hh.BC_ZERO_GRADIENT, xedges=xe)
+
+
+def test_batch_line_far_from_origin_rejected(hh):
+    """The batched stage kernel takes 1/width from 1/rx: a line whose edges lie more than 1e8
+    cell widths from the origin is refused before anything reaches the device."""
+    with pytest.raises(hh.HrwError, match="1e8 cell widths"):
+        hh.BurgersBatch([100, 100], [3, 3], [1e-3, 1e-3], xmin=[-1.0, 1e9], xmax=[1.0, 1e9 + 1.0])

@@ -116,6 +116,28 @@ def test_orders_and_k_exact(oracle, hh, dev, order, k):
     assert np.array_equal(u, ou)
 
 
+def test_grid_far_from_origin(oracle, hh, dev):
+    """A linear grid whose edges lie 1e12 cell widths from the origin (widths known to ~1e-4
+    only): the FAST form leaves the register kernels, whose 1/width starts from 1/rx, for the
+    tile kernel, and agrees with the oracle on the same widths."""
+    n = 1000
+    xmin, xmax = 1e9, 1e9 + 1.0
+    e = xmin + ((xmax - xmin) / n) * np.arange(n + 1)
+    w = e[1:] - e[:-1]
+    xc = np.linspace(-1.0, 1.0, n)
+    u0 = 0.5 + np.sin(np.pi * xc)
+    dt = 0.2 * w.min() / np.abs(u0).max()
+    ou, ot, steps, fe = oracle.burgers_rk(u0, w, tout=30 * dt, dt=dt)
+    for arith in ("exact", "fast"):
+        u, t, ode = run_burgers(hh, dev, u0, n=n, tout=30 * dt, dt=dt, xmin=xmin, xmax=xmax,
+                                arith=arith)
+        assert t == ot and ode.fevals == fe
+        if arith == "exact":
+            assert np.array_equal(u, ou)
+        else:
+            assert_close(u, ou)
+
+
 @pytest.mark.parametrize("flux", [0, 1])
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_nonuniform_grid_fused(oracle, hh, dev, k, flux):
