@@ -11,12 +11,13 @@
 //        12*beta_1(m+1) = 13 D^2 + 3 E^2
 //        12*beta_2(m+2) = 13 D^2 + 3 (2D - E)^2
 //    and the four candidate values are 6*c(:,r).T(m), r = -1..2 (used as vlr0(m), vrr0(m) =
-//    vlr1(m+1), vrr1(m+1) = vlr2(m+2), vrr2(m+2)); the factors 12 and 6 cancel in the
-//    normalised weights (eps is scaled by 12 to match);
+//    vlr1(m+1), vrr1(m+1) = vlr2(m+2), vrr2(m+2)), formed from one another and the smoothness
+//    terms; the factors 12 and 6 cancel in the normalised weights (eps is scaled by 12 to
+//    match);
 //  * one Lax-Friedrichs/Godunov flux and one grid edge per interface;
 //  * weights over a common denominator, w_r ~ d_r prod_{s!=r} q_s, with d scaled to integers
 //    (3, 6, 1) so they need three multiplies, and one v_rcp_f64 + two Newton steps per face
-//    instead of IEEE divides (also for 1/width).
+//    instead of IEEE divides; 1/width by one Newton step from 1/rx.
 //
 // All of this is an exact rewrite of the same algebra; results differ from the reference's
 // evaluation order by a few ulp per reconstruction (the FAST contract).
