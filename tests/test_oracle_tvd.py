@@ -215,3 +215,27 @@ def test_pbe_mass_balance(oracle):
                              dt=0.002)
     mass = lambda q: np.sum(q.reshape(n2, n1) * w1[None, :] * w2[:, None])  # noqa: E731
     assert fe >= 12 and abs(mass(u) - mass(u0)) <= 1e-13 * mass(u0)
+
+
+def test_burgers_cnu_composition(oracle):
+    """The non-uniform RHS composition given the uniform grid's own cnu (equal to the uniform
+    tables to ~1e-15) reproduces the uniform composition to 1e-12 for rktvd and mstvd: the table
+    reaches reconstruct with the right cell indexing.  A stretched grid's cnu, by contrast,
+    changes the result."""
+    n = 200
+    xe = np.linspace(-1.0, 1.0, n + 1)
+    w = xe[1:] - xe[:-1]
+    u0 = 0.5 + np.sin(np.pi * (xe[:-1] + xe[1:]) / 2)
+    for k in (2, 3):
+        cn = oracle.calc_cnu(k, xe)
+        a, ta, _, fa = oracle.burgers_rk(u0, w, k=k, bc=1, flux=1, tout=0.2, dt=2e-3, cnu=cn)
+        b, tb, _, fb = oracle.burgers_rk(u0, w, k=k, bc=1, flux=1, tout=0.2, dt=2e-3)
+        assert ta == tb and fa == fb and np.max(np.abs(a - b)) <= 1e-12
+        a, ta, fa = oracle.burgers_ms(u0, w, k=k, bc=1, flux=1, tout=0.2, dt=2e-3, cnu=cn)
+        b, tb, fb = oracle.burgers_ms(u0, w, k=k, bc=1, flux=1, tout=0.2, dt=2e-3)
+        assert ta == tb and fa == fb and np.max(np.abs(a - b)) <= 1e-12
+        s = np.linspace(0.0, 1.0, n + 1)
+        xs = -1.0 + 2.0 * (s + 0.5 * np.sin(2 * np.pi * s) / (2 * np.pi))
+        c, *_ = oracle.burgers_rk(u0, w, k=k, bc=1, flux=1, tout=0.2, dt=2e-3,
+                                  cnu=oracle.calc_cnu(k, xs))
+        assert np.max(np.abs(c - b)) > 1e-6
