@@ -165,9 +165,11 @@ def run_batch(args, torch, hh, W, dev):
     u0 = b.pack([c.u0 for c in cases], dev)
     u = u0.clone()
     r = b.tune(u)
+    form = b.config()[1]
     b.step(u, args.warmup, graph=True)
     ms = timed(torch, lambda: b.step(u, steps, graph=True))
-    log(f"batched (graph, {r} cells/lane): {steps} steps {ms:.3f} ms")
+    log(f"batched (graph, {'persistent' if form else 'chunk'} form, {r} cells/lane): "
+        f"{steps} steps {ms:.3f} ms")
     ms_nograph = timed(torch, lambda: b.step(u, steps, graph=False))
     log(f"batched (no graph): {steps} steps {ms_nograph:.3f} ms")
     # one-launch-per-line path on a bounded number of steps
@@ -177,11 +179,12 @@ def run_batch(args, torch, hh, W, dev):
     log(f"per-line: {pl_steps} steps {ms_pl:.3f} ms")
     value = total * steps / (ms / 1e3)
     per_line_value = total * pl_steps / (ms_pl / 1e3)
-    rl = roofline(64.0 / 3 * total, ms / (3 * steps),
-                  "hrw::batch_stage_kernel<LF,alphamax,%d>" % r, "batch_c3")
+    kname = "hrw::batch_pf_kernel" if form else "hrw::batch_stage_kernel"
+    rl = roofline(64.0 / 3 * total, ms / (3 * steps), "%s<LF,alphamax,%d>" % (kname, r), "batch_c3")
     config = {"workload": "burgers1d_batch_4096_lines_log_uniform_256_65536", "lines": len(cases),
               "cells": total, "k": "2 or 3 per line", "integrator": "rktvd order 3",
-              "cells_per_lane": r, "graph": True, "parallelism": "single GPU",
+              "cells_per_lane": r, "form": "persistent" if form else "chunk", "graph": True,
+              "parallelism": "single GPU",
               "batched_no_graph_cell_updates_per_s": total * steps / (ms_nograph / 1e3),
               "per_line_cell_updates_per_s": per_line_value, "per_line_steps": pl_steps,
               "speedup_vs_per_line": value / per_line_value}

@@ -1145,12 +1145,13 @@ extern "C" int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tou
 // captured hipGraph.  The plain path (one set of launches per line) is kept for comparison.
 struct hrw_batch_s {
     int nlines = 0, numflux = HRW_FLUX_LAX_FRIEDRICHS, arith = HRW_ARITH_FAST, R = 8;
+    int form = HRW_BATCH_CHUNK;
     double eps = 1e-6, alpha = -1.0;
     int64_t total = 0;
     std::vector<int64_t> h_off;
     std::vector<int> h_n, h_k;
     std::vector<double> h_dt, h_xmin, h_rx;
-    long long nchunks = 0;
+    long long nchunks = 0, ninterior = 0;
     int chunk_R = 0;
     long long *d_off = nullptr;
     int *d_n = nullptr, *d_k = nullptr, *d_cline = nullptr, *d_cstart = nullptr;
@@ -1162,7 +1163,7 @@ struct hrw_batch_s {
     bool own_stream = false;
     hipGraphExec_t graph = nullptr;
     const double *graph_u = nullptr;
-    int graph_R = 0;
+    int graph_R = 0, graph_form = 0;
     bool alphamax() const { return numflux == HRW_FLUX_LAX_FRIEDRICHS && alpha < 0; }
     ~hrw_batch_s()
     {
@@ -1181,16 +1182,22 @@ template <class T> static int upload(T **d, const std::vector<T> &h)
     return HRW_OK;
 }
 
-// Chunk table for chunks of 62*R cells (the output cells of one wavefront).
+// Chunk table for chunks of 62*R cells (the output cells of one wavefront): first every chunk
+// whose wavefront reads only cells of its line for K = 3 (the interior chunks, which need no
+// boundary handling for any k), line by line, then the others.
 static int batch_chunks(hrw_batch_s *b, int R)
 {
     if (b->chunk_R == R) return HRW_OK;
-    std::vector<int> cl, cs;
+    std::vector<int> cl, cs, bl, bs;
     for (int l = 0; l < b->nlines; ++l)
         for (int st = 0; st < b->h_n[l]; st += kOutLanes * R) {
-            cl.push_back(l);
-            cs.push_back(st);
+            const bool interior = (long long)st - R - 2 >= 0 && (long long)st + 63LL * R + 2 <= b->h_n[l];
+            (interior ? cl : bl).push_back(l);
+            (interior ? cs : bs).push_back(st);
         }
+    b->ninterior = (long long)cl.size();
+    cl.insert(cl.end(), bl.begin(), bl.end());
+    cs.insert(cs.end(), bs.begin(), bs.end());
     dfree(b->d_cline);
     dfree(b->d_cstart);
     CALL(upload(&b->d_cline, cl));
@@ -1322,6 +1329,7 @@ static int batch_stage_sets(hrw_batch_s *b, const unsigned long long *ain,
     a.chunk_line = b->d_cline;
     a.chunk_start = b->d_cstart;
     a.nchunks = b->nchunks;
+    a.ninterior = b->ninterior;
     a.nlines = b->nlines;
     a.stage = stage;
     a.eps = b->eps;
@@ -1333,7 +1341,7 @@ static int batch_stage_sets(hrw_batch_s *b, const unsigned long long *ain,
         a.alpha_clear = aclear;
     }
     const bool lf = b->numflux == HRW_FLUX_LAX_FRIEDRICHS;
-    launch_batch_stage(lf ? LF : GODUNOV, lf && amax, b->chunk_R, a, b->stream);
+    launch_batch_stage(lf ? LF : GODUNOV, lf && amax, b->chunk_R, b->form, a, b->stream);
     LAUNCH_CHECK();
     return HRW_OK;
 }
@@ -1366,7 +1374,7 @@ extern "C" int hrw_batch_step(hrw_batch_t b, double *u, int64_t nsteps, double *
     if (nsteps == 0) return HRW_OK;
     CALL(batch_seed(b, u));
     if (use_graph) {
-        if (!b->graph || b->graph_u != u || b->graph_R != b->chunk_R) {
+        if (!b->graph || b->graph_u != u || b->graph_R != b->chunk_R || b->graph_form != b->form) {
             if (b->graph) {
                 (void)hipGraphExecDestroy(b->graph);
                 b->graph = nullptr;
@@ -1382,6 +1390,7 @@ extern "C" int hrw_batch_step(hrw_batch_t b, double *u, int64_t nsteps, double *
             if (e != hipSuccess) return fail(HRW_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
             b->graph_u = u;
             b->graph_R = b->chunk_R;
+            b->graph_form = b->form;
         }
         for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
     } else {
@@ -1433,8 +1442,8 @@ extern "C" int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps,
     return HRW_OK;
 }
 
-// Autotuner: times the batched RK3 step for each candidate chunk width on a scratch copy of
-// the state and keeps the fastest (the result is the same for every candidate).
+// Autotuner: times the batched RK3 step for each kernel form and chunk width on a scratch copy
+// of the state and keeps the fastest (the result is the same for every candidate).
 extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
 {
     if (!b) return fail(HRW_ERR_ARG, "null batch");
@@ -1443,9 +1452,11 @@ extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int best = b->chunk_R;
+    int best = b->chunk_R, best_form = b->form;
     float best_ms = 1e30f;
-    for (int R : {4, 6, 8}) {
+    for (int cand = 0; cand < 6; ++cand) {
+        const int R = 4 + 2 * (cand % 3);
+        b->form = cand / 3;
         int rc = batch_chunks(b, R);
         if (rc == HRW_OK) {
             (void)hipMemcpyAsync(scratch, u, b->total * sizeof(double), hipMemcpyDeviceToDevice, b->stream);
@@ -1465,12 +1476,14 @@ extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
         if (ms < best_ms) {
             best_ms = ms;
             best = R;
+            best_form = b->form;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dfree(scratch);
     b->R = best;
+    b->form = best_form;
     CALL(batch_chunks(b, best));
     if (chosen) *chosen = best;
     return HRW_OK;
@@ -1483,6 +1496,23 @@ extern "C" int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane)
         return fail(HRW_ERR_ARG, "cells_per_lane must be 4, 6 or 8");
     b->R = cells_per_lane;
     return batch_chunks(b, cells_per_lane);
+}
+
+extern "C" int hrw_batch_set_form(hrw_batch_t b, int form)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (form != HRW_BATCH_CHUNK && form != HRW_BATCH_PERSISTENT)
+        return fail(HRW_ERR_ARG, "form must be HRW_BATCH_CHUNK (0) or HRW_BATCH_PERSISTENT (1)");
+    b->form = form;
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_config(hrw_batch_t b, int *cells_per_lane, int *form)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    if (cells_per_lane) *cells_per_lane = b->chunk_R;
+    if (form) *form = b->form;
+    return HRW_OK;
 }
 
 // ------------------------------------------------------------------ adjoint -----------

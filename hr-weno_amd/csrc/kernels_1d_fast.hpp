@@ -559,8 +559,9 @@ void stage1d_fast_kernel(Stage1DArgs a)
 // --------------------------------------------------------------------------------------
 // Many independent periodic lines in one launch.  Line l: cells [off[l], off[l] + n[l]) of
 // the flat state, WENO order k[l], step h[l], linear grid xmin[l] + rx[l]*i.  Chunk c of the
-// table covers cells [start[c], start[c] + 62R) of line line[c]; wavefront c of the grid takes
-// chunk c.  Lax-Friedrichs alpha = max|v| per line: slot set s holds one partial per line.
+// table covers cells [start[c], start[c] + 62R) of line line[c]; in batch_stage_kernel
+// wavefront c of the grid takes chunk c (batch_pf_kernel below walks the table).
+// Lax-Friedrichs alpha = max|v| per line: slot set s holds one partial per line.
 struct BatchArgs {
     const double *us, *u0;
     double *uo;
@@ -574,7 +575,25 @@ struct BatchArgs {
     const unsigned long long *alpha_in;   // nlines partials, read
     unsigned long long *alpha_out;        // nlines partials, atomicMax
     unsigned long long *alpha_clear;      // nlines partials, cleared
+    long long ninterior;                  // chunks [0, ninterior) are interior for K = 3
 };
+
+// Line `line` of the batch as single-line stage arguments.
+__device__ __forceinline__ Stage1DArgs batch_line_args(const BatchArgs &b, int line)
+{
+    const long long off = b.off[line];
+    Stage1DArgs a{};
+    a.us = b.us + off;
+    a.u0 = b.u0 + off;
+    a.uo = b.uo + off;
+    a.n = b.n[line];
+    a.xmin = b.xmin[line];
+    a.rx = b.rx[line];
+    a.eps = b.eps;
+    a.h = b.h[line];
+    a.stage = b.stage;
+    return a;
+}
 
 template <int FLUX, bool ALPHAMAX, int R, int NPW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW))))
@@ -590,17 +609,7 @@ void batch_stage_kernel(BatchArgs b)
     }
     if (wave >= b.nchunks) return;   // whole wavefronts leave together
     const int line = b.chunk_line[wave];
-    const long long off = b.off[line];
-    Stage1DArgs a{};
-    a.us = b.us + off;
-    a.u0 = b.u0 + off;
-    a.uo = b.uo + off;
-    a.n = b.n[line];
-    a.xmin = b.xmin[line];
-    a.rx = b.rx[line];
-    a.eps = b.eps;
-    a.h = b.h[line];
-    a.stage = b.stage;
+    const Stage1DArgs a = batch_line_args(b, line);
     double alpha = b.alpha;
     if constexpr (ALPHAMAX) alpha = __longlong_as_double((long long)b.alpha_in[line]);
     const int W = b.chunk_start[wave];
@@ -613,6 +622,99 @@ void batch_stage_kernel(BatchArgs b)
     if constexpr (ALPHAMAX) {
         m = wave_max(m);
         if (lane == 0) atomicMax(b.alpha_out + line, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// Persistent form of the batch: as many wavefronts as fit on the device, wave w taking chunks
+// w, w + N, ... of the table (N waves in the grid), whose first ninterior chunks are interior
+// for K = 3 and so for every k.  An interior chunk's loads (with the K = 3 halo, of which a line
+// of k < 3 uses part) are issued into a second register set before the previous chunk's
+// arithmetic, unconditionally (the last iteration reloads its own chunk); the chunks that are
+// not interior follow, one per iteration, with their own loads.  (The second register set
+// costs a wave per SIMD against batch_stage_kernel; which form is faster depends on the
+// ensemble, and hrw_batch_tune times both.)
+template <int K, int R, int NPW>
+__device__ __forceinline__ ChunkData<K, R, NPW> narrow_chunk(const ChunkData<3, R, NPW> &d)
+{
+    ChunkData<K, R, NPW> o;
+#pragma unroll
+    for (int i = 0; i < R; ++i) o.run[i] = d.run[i];
+    if constexpr (NPW >= 1) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) o.p1[i] = d.p1[i];
+    }
+    if constexpr (NPW >= 2) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) o.p2[i] = d.p2[i];
+    }
+    // K = 3 halo: cells W-R-2, W-R-1 (lanes 0..31) and W+63R, W+63R+1 (lanes 32..63)
+    if constexpr (K == 2) o.halo[0] = (threadIdx.x & 63) < 32 ? d.halo[1] : d.halo[0];
+    if constexpr (K == 3) {
+        o.halo[0] = d.halo[0];
+        o.halo[1] = d.halo[1];
+    }
+    return o;
+}
+
+template <int FLUX, bool ALPHAMAX, int R, int NPW>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW) > 2 ? fast_waves(R, NPW) - 1 : 2)))
+void batch_pf_kernel(BatchArgs b)
+{
+    const long long nw = (long long)gridDim.x * (kBlock / 64);
+    const long long w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
+    const int lane = threadIdx.x & 63;
+    if (ALPHAMAX && b.alpha_clear) {
+        for (long long l = (long long)blockIdx.x * kBlock + threadIdx.x; l < b.nlines;
+             l += (long long)gridDim.x * kBlock)
+            b.alpha_clear[l] = 0ull;
+    }
+    auto publish = [&](int line, double m) {
+        if constexpr (ALPHAMAX) {
+            m = wave_max(m);
+            if (lane == 0) atomicMax(b.alpha_out + line, (unsigned long long)__double_as_longlong(m));
+        }
+    };
+    auto line_alpha = [&](int line) {
+        return ALPHAMAX ? __longlong_as_double((long long)b.alpha_in[line]) : b.alpha;
+    };
+    if (w < b.ninterior) {
+        int line = b.chunk_line[w];
+        Stage1DArgs a = batch_line_args(b, line);
+        int W = b.chunk_start[w];
+        ChunkData<3, R, NPW> cur, nxt;
+        load_chunk<3, R, NPW>(a, W, cur);
+        for (long long i = w; i < b.ninterior; i += nw) {
+            const long long i1 = i + nw < b.ninterior ? i + nw : i;
+            const int line1 = b.chunk_line[i1];
+            const Stage1DArgs a1 = batch_line_args(b, line1);
+            const int W1 = b.chunk_start[i1];
+            load_chunk<3, R, NPW>(a1, W1, nxt);
+            const double alpha = line_alpha(line);
+            double m;
+            switch (b.k[line]) {
+            case 1: m = chunk_stage<1, FLUX, true, R, NPW, true>(a, W, alpha, narrow_chunk<1>(cur)); break;
+            case 2: m = chunk_stage<2, FLUX, true, R, NPW, true>(a, W, alpha, narrow_chunk<2>(cur)); break;
+            default: m = chunk_stage<3, FLUX, true, R, NPW, true>(a, W, alpha, cur); break;
+            }
+            publish(line, m);
+            cur = nxt;
+            line = line1;
+            a = a1;
+            W = W1;
+        }
+    }
+    for (long long i = b.ninterior + w; i < b.nchunks; i += nw) {
+        const int line = b.chunk_line[i];
+        const Stage1DArgs a = batch_line_args(b, line);
+        const int W = b.chunk_start[i];
+        const double alpha = line_alpha(line);
+        double m;
+        switch (b.k[line]) {
+        case 1: m = chunk_load_stage<1, FLUX, true, R, NPW>(a, W, alpha); break;
+        case 2: m = chunk_load_stage<2, FLUX, true, R, NPW>(a, W, alpha); break;
+        default: m = chunk_load_stage<3, FLUX, true, R, NPW>(a, W, alpha); break;
+        }
+        publish(line, m);
     }
 }
 

@@ -24,10 +24,9 @@ inline int device_cus()
     return cus;
 }
 
-// Register-blocked 1D stage (FAST form): r = cells per lane (4, 6, 8), negative for the
-// persistent prefetching kernel.  One definition per K (launch_fast_k<K>.o).
-// form: one chunk per wavefront (stage1d_fast_kernel) or persistent with the next chunk's
-// loads issued ahead (stage1d_pf_kernel); r cells per lane.
+// Register-blocked 1D stage (FAST form).  form: one chunk per wavefront (stage1d_fast_kernel)
+// or persistent with the next chunk's loads issued ahead (stage1d_pf_kernel); r cells per lane
+// (4, 6, 8).  One definition per K (launch_fast_k<K>.o).
 enum { FORM_CHUNK = 0, FORM_PERSISTENT = 1 };
 void launch_fast_1d_k1(int flux, bool periodic, bool alphamax, int form, int r, const Stage1DArgs &a, hipStream_t s);
 void launch_fast_1d_k2(int flux, bool periodic, bool alphamax, int form, int r, const Stage1DArgs &a, hipStream_t s);
@@ -37,8 +36,8 @@ void launch_tile_1d(int k, int flux, bool periodic, bool fast, bool alphamax, co
                     hipStream_t s);
 // 2D population balance stage.
 void launch_pbe2d(int k, int bc, bool fast, const Stage2DArgs &a, hipStream_t s);
-// One stage of every line of a batch.
-void launch_batch_stage(int flux, bool alphamax, int R, const BatchArgs &a, hipStream_t s);
+// One stage of every line of a batch (form HRW_BATCH_CHUNK or HRW_BATCH_PERSISTENT).
+void launch_batch_stage(int flux, bool alphamax, int R, int form, const BatchArgs &a, hipStream_t s);
 // The reverse of one stage.
 void launch_adjoint_stage(int R, bool alphamax, const AdjArgs &g, hipStream_t s);
 

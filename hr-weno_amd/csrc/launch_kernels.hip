@@ -80,9 +80,41 @@ static void lbatch_r(const BatchArgs &a, hipStream_t s)
     default: batch_stage_kernel<FLUX, AMAX, R, 0><<<grid, kBlock, 0, s>>>(a); break;
     }
 }
-template <int FLUX, bool AMAX>
-static void lbatch(int R, const BatchArgs &a, hipStream_t s)
+// Persistent form: as many blocks as fit on the device at once (occupancy of this
+// instantiation, asked once), never more than there are chunks.
+template <int FLUX, bool AMAX, int R, int NPW>
+static void lbatch_pf_n(const BatchArgs &a, hipStream_t s)
 {
+    static int per_cu = 0;
+    if (!per_cu) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void *>(&batch_pf_kernel<FLUX, AMAX, R, NPW>), kBlock, 0) != hipSuccess ||
+            nb <= 0)
+            nb = 2;
+        per_cu = nb;
+    }
+    const long long want = (a.nchunks + kBlock / 64 - 1) / (kBlock / 64);
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(want, (long long)per_cu * device_cus()));
+    batch_pf_kernel<FLUX, AMAX, R, NPW><<<grid, kBlock, 0, s>>>(a);
+}
+template <int FLUX, bool AMAX, int R>
+static void lbatch_pf_r(const BatchArgs &a, hipStream_t s)
+{
+    switch (stage_npw(a.stage)) {
+    case 1: lbatch_pf_n<FLUX, AMAX, R, 1>(a, s); break;
+    default: lbatch_pf_n<FLUX, AMAX, R, 0>(a, s); break;
+    }
+}
+template <int FLUX, bool AMAX>
+static void lbatch(int R, int form, const BatchArgs &a, hipStream_t s)
+{
+    if (form == HRW_BATCH_PERSISTENT) {
+        if (R == 4) lbatch_pf_r<FLUX, AMAX, 4>(a, s);
+        else if (R == 6) lbatch_pf_r<FLUX, AMAX, 6>(a, s);
+        else lbatch_pf_r<FLUX, AMAX, 8>(a, s);
+        return;
+    }
     if (R == 4) lbatch_r<FLUX, AMAX, 4>(a, s);
     else if (R == 6) lbatch_r<FLUX, AMAX, 6>(a, s);
     else lbatch_r<FLUX, AMAX, 8>(a, s);
@@ -105,11 +137,11 @@ void launch_pbe2d(int k, int bc, bool fast, const Stage2DArgs &a, hipStream_t s)
     else l2d_b<3>(bc, fast, a, s);
 }
 
-void launch_batch_stage(int flux, bool alphamax, int R, const BatchArgs &a, hipStream_t s)
+void launch_batch_stage(int flux, bool alphamax, int R, int form, const BatchArgs &a, hipStream_t s)
 {
-    if (flux == LF && alphamax) lbatch<LF, true>(R, a, s);
-    else if (flux == LF) lbatch<LF, false>(R, a, s);
-    else lbatch<GODUNOV, false>(R, a, s);
+    if (flux == LF && alphamax) lbatch<LF, true>(R, form, a, s);
+    else if (flux == LF) lbatch<LF, false>(R, form, a, s);
+    else lbatch<GODUNOV, false>(R, form, a, s);
 }
 
 }  // namespace hrw

@@ -78,6 +78,8 @@ SIGNATURES = {
     "hrw_batch_step_per_line": [P, P, L, PD],
     "hrw_batch_tune": [P, P, PI],
     "hrw_batch_set_cells_per_lane": [P, I],
+    "hrw_batch_set_form": [P, I],
+    "hrw_batch_config": [P, PI, PI],
     "hrw_adjoint_create": [P, I, D, L, I, C.POINTER(P)],
     "hrw_batch_adjoint_create": [P, L, I, C.POINTER(P)],
     "hrw_adjoint_destroy": [P],

@@ -426,6 +426,16 @@ class BurgersBatch:
     def set_cells_per_lane(self, r: int):
         call("hrw_batch_set_cells_per_lane", self._h, int(r))
 
+    def set_form(self, form: int):
+        """0: one wavefront per chunk; 1: persistent wavefronts walking the chunk table."""
+        call("hrw_batch_set_form", self._h, int(form))
+
+    def config(self):
+        """(cells per lane, form) of the stage kernel."""
+        r, f = C.c_int(), C.c_int()
+        call("hrw_batch_config", self._h, C.byref(r), C.byref(f))
+        return r.value, f.value
+
     def step(self, u, nsteps: int, graph: bool = True):
         cur = self._enter()
         call("hrw_batch_step", self._h, _dptr(u), int(nsteps), _pd(self.t), int(bool(graph)))

@@ -219,10 +219,16 @@ int hrw_batch_set_stream(hrw_batch_t b, hrw_stream_t stream);
 int hrw_batch_step(hrw_batch_t b, double *u, int64_t nsteps, double *t, int use_graph);
 /* The same, each line advanced by its own launches (the one-launch-per-line path). */
 int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps, double *t);
-/* Autotuner: times each chunk width (cells per lane 4 or 8) on a scratch copy of u and keeps
- * the fastest; or set it directly. */
+/* Stage kernel of the batch: one wavefront per chunk of 62*cells_per_lane cells (CHUNK), or
+ * one wavefront per SIMD slot walking the chunk table with the next chunk's loads issued ahead
+ * (PERSISTENT).  Results are bit-identical for every form and width. */
+enum { HRW_BATCH_CHUNK = 0, HRW_BATCH_PERSISTENT = 1 };
+/* Autotuner: times each form and chunk width (cells per lane 4, 6, 8) on a scratch copy of u
+ * and keeps the fastest (cells_per_lane: the width chosen, may be NULL); or set them directly. */
 int hrw_batch_tune(hrw_batch_t b, const double *u, int *cells_per_lane);
 int hrw_batch_set_cells_per_lane(hrw_batch_t b, int cells_per_lane);
+int hrw_batch_set_form(hrw_batch_t b, int form);
+int hrw_batch_config(hrw_batch_t b, int *cells_per_lane, int *form);
 
 /* ---------------------------------------------------------------- discrete adjoint - */
 /* Reverse mode of nsteps fixed rktvd steps (src/hrweno_tvdode.f90:136-171) of periodic 1D

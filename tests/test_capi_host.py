@@ -111,3 +111,14 @@ def test_batch_line_far_from_origin_rejected(hh):
     cell widths from the origin is refused before anything reaches the device."""
     with pytest.raises(hh.HrwError, match="1e8 cell widths"):
         hh.BurgersBatch([100, 100], [3, 3], [1e-3, 1e-3], xmin=[-1.0, 1e9], xmax=[1.0, 1e9 + 1.0])
+
+
+def test_batch_form_and_config_need_a_batch(hh):
+    """The batch form setters check their handle before anything else."""
+    from hrweno_hip import _lib
+    lib = _lib.lib()
+    r, f = C.c_int(-1), C.c_int(-1)
+    assert lib.hrw_batch_set_form(None, 1) != 0
+    assert b"null batch" in lib.hrw_last_error()
+    assert lib.hrw_batch_config(None, C.byref(r), C.byref(f)) != 0
+    assert r.value == -1 and f.value == -1

@@ -59,17 +59,46 @@ def test_tune_keeps_results(hh, dev):
     import torch
     lens, ks, dts, u0s, widths = ensemble(24, 5)
     res = []
-    for r in (4, 6, 8):
-        b = hh.BurgersBatch(lens, ks, dts)
-        u = b.pack(u0s, dev)
-        b.set_cells_per_lane(r)
-        b.step(u, 4, graph=True)
-        torch.cuda.synchronize()
-        res.append(u.cpu().numpy())
-    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+    for form in (0, 1):
+        for r in (4, 6, 8):
+            b = hh.BurgersBatch(lens, ks, dts)
+            u = b.pack(u0s, dev)
+            b.set_cells_per_lane(r)
+            b.set_form(form)
+            assert b.config() == (r, form)
+            b.step(u, 4, graph=True)
+            torch.cuda.synchronize()
+            res.append(u.cpu().numpy())
+    for x in res[1:]:
+        assert np.array_equal(res[0], x)
     b = hh.BurgersBatch(lens, ks, dts)
     u = b.pack(u0s, dev)
     assert b.tune(u) in (4, 6, 8)
+    assert b.config()[1] in (0, 1)
+
+
+@pytest.mark.parametrize("flux,alpha", [(0, -1.0), (0, 1.5), (1, -1.0)])
+@pytest.mark.parametrize("form", [0, 1])
+def test_forms_match_per_line_every_flux(hh, dev, flux, alpha, form):
+    """Both batch kernels (one wavefront per chunk; persistent wavefronts with the next chunk's
+    loads issued ahead, interior chunks first) give the per-line path's bits for every flux,
+    on lines from 1 cell to a few dozen chunks."""
+    import torch
+    lens, ks, dts, u0s, widths = ensemble(30, 11, nmax=12000)
+    for r in (4, 6, 8):
+        outs = []
+        for per_line in (False, True):
+            b = hh.BurgersBatch(lens, ks, dts, flux=flux, alpha=alpha)
+            b.set_cells_per_lane(r)
+            b.set_form(form)
+            u = b.pack(u0s, dev)
+            if per_line:
+                b.step_per_line(u, 3)
+            else:
+                b.step(u, 3, graph=True)
+            torch.cuda.synchronize()
+            outs.append(u)
+        assert torch.equal(outs[0], outs[1]), (r, form)
 
 
 def test_config3_subset(oracle, hh, dev):
@@ -87,8 +116,13 @@ def test_config3_subset(oracle, hh, dev):
     b2 = hh.BurgersBatch(lens, ks, dts)
     u2 = b2.pack([c.u0 for c in cases], dev)
     b2.step_per_line(u2, 10)
+    b3 = hh.BurgersBatch(lens, ks, dts)
+    b3.set_form(1)
+    u3 = b3.pack([c.u0 for c in cases], dev)
+    b3.step(u3, 10, graph=True)
     torch.cuda.synchronize()
     assert torch.equal(u1, u2)
+    assert torch.equal(u3, u2)
     out = b1.unpack(u1)
     for l in (0, 7, 31, 63):
         c = cases[l]
