@@ -5,7 +5,9 @@
 //
 //  * each lane owns R consecutive cells; the K-1... halo comes from the neighbouring lanes by
 //    ds_bpermute shuffles (LDS pipe, no VALU), lanes 0 and 63 load theirs directly;
-//  * adjacent cells share work: with D = v0 - 2v1 + v2 and E = v0 - v2 on the triple
+//  * adjacent cells share work (and, for K = 3 with R = 4, adjacent lanes: a lane takes its
+//    left neighbour's last two triples by shuffles instead of forming them again): with
+//    D = v0 - 2v1 + v2 and E = v0 - v2 on the triple
 //    T(m) = (v_m, v_m+1, v_m+2), the three smoothness indicators drawing on T(m) are
 //        12*beta_0(m)   = 13 D^2 + 3 (E + 2D)^2
 //        12*beta_1(m+1) = 13 D^2 + 3 E^2
@@ -337,9 +339,96 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         }
     };
 
+    if constexpr (K == 3 && R == 4) {
+        // ---- march over the triples T(c0+t), t = 0..R-1, reconstructing cells 2..R-1 as their
+        // three triples are formed; T(c0-2) and T(c0-1) are the left neighbour's last two
+        // triples: the parts cells 0 and 1 use come over by lane shuffles after the march (the
+        // same values a lane would form from its own window, so the results are those of the
+        // march below), and those two cells follow.  Two triples (36 FP64 operations) fewer per
+        // lane; with R = 6 or 8 the values held across the march no longer fit the register
+        // budget of the occupancy those kernels run at, so they take the march below.
+        const double eps12 = 12.0 * a.eps;
+        const Tri3 ta = tri3(v[2], v[3], v[4]);   // T(c0)
+        const Tri3 tb = tri3(v[3], v[4], v[5]);   // T(c0+1)
+        Tri3 t2 = ta, t1 = tb;
+        double vl2 = 0.0, vr_prev = 0.0, f_first = 0.0, fl = 0.0;
+        double un3 = 0.0, L3 = 0.0, unR2 = 0.0, LR2 = 0.0, un_pair = 0.0, L_pair = 0.0;
+#pragma unroll
+        for (int t = 2; t < R; ++t) {
+            const Tri3 t0 = tri3(v[t + 2], v[t + 3], v[t + 4]);   // T(c0+t)
+            double vl, vr;
+            recon3(eps12, t2, t1, t0, vl, vr);
+            t2 = t1;
+            t1 = t0;
+            if (t == 2) {
+                vl2 = vl;
+                vr_prev = vr;
+                continue;
+            }
+            const double f = burgers_numflux<FLUX, true>(vr_prev, vl, alpha);   // face c0+t-1/2
+            vr_prev = vr;
+            if (t == 3) {
+                f_first = f;
+                fl = f;
+                continue;
+            }
+            // cell t-1 (3 <= t-1 <= R-2) between fl and f
+            const int c = t - 1;
+            double un, L;
+            finish(c, fl, f, width_of(c0 + c, edge(c0 + c), edge(c0 + c + 1)), un, L);
+            fl = f;
+            if (c == 3) {
+                un3 = un;
+                L3 = L;
+            } else if (c == R - 2) {
+                unR2 = un;
+                LR2 = L;
+            } else if (c % 2 == 0) {
+                un_pair = un;
+                L_pair = L;
+            } else {
+                store_pair(c - 1, un_pair, un, L_pair, L);
+            }
+        }
+        // ---- T(c0-2) (bC, P2, P1) and T(c0-1) (bB, P1, P0, bC, P2) from lane-1, cells 0 and 1
+        Tri3 l2{}, l1{};
+        l2.bC = __shfl_up(t2.bC, 1, 64);
+        l2.P2 = __shfl_up(t2.P2, 1, 64);
+        l2.P1 = __shfl_up(t2.P1, 1, 64);
+        l1.bB = __shfl_up(t1.bB, 1, 64);
+        l1.P1 = __shfl_up(t1.P1, 1, 64);
+        l1.P0 = __shfl_up(t1.P0, 1, 64);
+        l1.bC = __shfl_up(t1.bC, 1, 64);
+        l1.P2 = __shfl_up(t1.P2, 1, 64);
+        const double vr_left = __shfl_up(vr_prev, 1, 64);   // vr of cell c0-1 (lane-1's last)
+        double vl0, vr0, vl1, vr1;
+        recon3(eps12, l2, l1, ta, vl0, vr0);
+        recon3(eps12, l1, ta, tb, vl1, vr1);
+        const double vl_right = __shfl_down(vl0, 1, 64);    // vl of cell c0+R (lane+1's first)
+        const double f0 = burgers_numflux<FLUX, true>(vr_left, vl0, alpha);   // face c0-1/2
+        const double f1 = burgers_numflux<FLUX, true>(vr0, vl1, alpha);       // face c0+1/2
+        const double f2 = burgers_numflux<FLUX, true>(vr1, vl2, alpha);       // face c0+3/2
+        const double fR = burgers_numflux<FLUX, true>(vr_prev, vl_right, alpha);   // c0+R-1/2
+        const double e1 = edge(c0 + 1), e2 = edge(c0 + 2);
+        double un0, L0, un1, L1, un2, L2;
+        finish(0, f0, f1, width_of(c0, edge(c0), e1), un0, L0);
+        finish(1, f1, f2, width_of(c0 + 1, e1, e2), un1, L1);
+        finish(2, f2, f_first, width_of(c0 + 2, e2, edge(c0 + 3)), un2, L2);
+        store_pair(0, un0, un1, L0, L1);
+        double unl, Ll;
+        finish(R - 1, fl, fR, width_of(c0 + R - 1, edge(c0 + R - 1), edge(c0 + R)), unl, Ll);
+        if constexpr (R == 4) {
+            store_pair(2, un2, unl, L2, Ll);
+        } else {
+            store_pair(2, un2, un3, L2, L3);
+            store_pair(R - 2, unR2, unl, LR2, Ll);
+        }
+        return owned ? m_out : 0.0;
+    }
+
     // ---- march over the lane's cells t = 0..R-1: reconstruction of cell c0+t; the interface
     // between cells t-1 and t; cell t-1 finished for t >= 2
-    const double eps12 = 12.0 * a.eps;
+    [[maybe_unused]] const double eps12 = 12.0 * a.eps;
     [[maybe_unused]] Tri3 t2, t1;
     [[maybe_unused]] Pair2 pp1;
     if constexpr (K == 3) {
@@ -511,7 +600,9 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
     if constexpr (ALPHAMAX) publish_max(m, a.alpha_out, s_red);
 }
 
-// (The second register set of the prefetch costs one wave per SIMD against the plain form.)
+// (The second register set of the prefetch costs one wave per SIMD against the plain form.
+// With the shared triples K = 3, R = 4 would fit four with at most one pointwise operand; it
+// measured no faster than three.)
 template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW) > 2 ? fast_waves(R, NPW) - 1 : 2)))
 void stage1d_pf_kernel(Stage1DArgs a)
