@@ -14,7 +14,9 @@ replayed) vs one launch per line; `pbe` -- 2D population balance on 2048 x 2048 
 
 Prints ONE JSON line.  `roofline` prices the dominant kernel against HBM with the algorithmic
 bytes per cell-step (RK3 1D: 16 + 24 + 24 = 64; multistep 2D: 40); `cpu_baseline` times the
-reference's own compiled hrweno_weno/hrweno_fluxes (oracle/_ref) on a bounded sample.
+reference's own compiled hrweno_weno/hrweno_fluxes (oracle/_ref) on a bounded sample on one
+core, and beside it (`all_cores`, labelled apart) the oracle's C restatement with OpenMP on every
+core the job may use, with the host's CPU model and core counts (`host`).
 """
 from __future__ import annotations
 
@@ -78,9 +80,51 @@ def cpu_baseline(seconds: float):
     th.join()
     threading.stack_size(old_size)
     steps, dt = res["steps"], res["time"]
-    return {"value": n * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{n} cells x {steps} RK3 steps of the same workload (WENO5-LF, periodic), "
-                      f"{dt:.1f} s on one host core; {what}"}
+    out = {"value": n * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+           "sample": f"{n} cells x {steps} RK3 steps of the same workload (WENO5-LF, periodic), "
+                     f"{dt:.1f} s on one host core; {what}"}
+    out["host"] = host_info()
+    out["all_cores"] = cpu_all_cores(seconds)
+    return out
+
+
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_all_cores(seconds: float):
+    """The oracle's C restatement (OpenMP over cells, the reference's own parallel pattern of
+    example2:95-114 applied to one line) on every host core this job may use, on a 2**22-cell
+    sample of the headline workload; labelled apart from the one-core reference figure."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    from hrweno_hip import workloads as W
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    n = 1 << 22
+    c = W.config2(n=n)
+    o = O.Oracle()
+    o.set_threads(threads)
+    run = lambda steps: o.burgers_rk(c.u0, c.width, tout=W.steps_tout(0.0, c.dt, steps),  # noqa: E731
+                                     dt=c.dt)
+    t0 = time.perf_counter()
+    run(2)
+    steps = max(2, int(seconds / ((time.perf_counter() - t0) / 2)))
+    t0 = time.perf_counter()
+    run(steps)
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} cells x {steps} RK3 steps, {dt:.1f} s on {threads} threads; oracle C "
+                      "restatement (gcc -O3, no FMA, OpenMP)"}
 
 
 def traffic_for(kernel_key: str):

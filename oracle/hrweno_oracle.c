@@ -151,6 +151,7 @@ int hro_weno_reconstruct(int n, int k, double eps, const double *cnu,
     /* vext(1-km : n+km), :171-173 (constant extrapolation) */
     double *vext = (double *)malloc((size_t)(n + 2 * km) * sizeof(double));
     if (!vext) return -2;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
     for (int64_t i = 0; i < n; ++i) vext[i + km] = v[i * incv];
     for (int i = 0; i < km; ++i) {
         vext[i] = v[0];
@@ -294,6 +295,7 @@ void hro_rktvd_integrate(hro_ode *ode, hro_rhs_fn fu, void *ctx, double *u,
     if (ode->order == 1) {                            /* :136-143 */
         for (;;) {
             fu(ctx, *t, u, udot);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
             for (int64_t i = 0; i < n; ++i) u[i] = u[i] + dt * udot[i];
             *t = *t + dt;
             ode->fevals += 1;
@@ -302,8 +304,10 @@ void hro_rktvd_integrate(hro_ode *ode, hro_rhs_fn fu, void *ctx, double *u,
     } else if (ode->order == 2) {                     /* :147-156 */
         for (;;) {
             fu(ctx, *t, u, udot);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
             for (int64_t i = 0; i < n; ++i) ui[i] = u[i] + dt * udot[i];
             fu(ctx, *t + dt, ui, udot);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
             for (int64_t i = 0; i < n; ++i) u[i] = ((u[i] + ui[i]) + dt * udot[i]) / 2;
             *t = *t + dt;
             ode->fevals += 2;
@@ -312,10 +316,13 @@ void hro_rktvd_integrate(hro_ode *ode, hro_rhs_fn fu, void *ctx, double *u,
     } else {                                          /* :160-171 */
         for (;;) {
             fu(ctx, *t, u, udot);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
             for (int64_t i = 0; i < n; ++i) ui[i] = u[i] + dt * udot[i];
             fu(ctx, *t + dt, ui, udot);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
             for (int64_t i = 0; i < n; ++i) ui[i] = ((3 * u[i] + ui[i]) + dt * udot[i]) / 4;
             fu(ctx, *t + dt / 2, ui, udot);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
             for (int64_t i = 0; i < n; ++i)
                 u[i] = ((u[i] + 2 * ui[i]) + (2 * dt) * udot[i]) / 3;
             *t = *t + dt;
@@ -351,6 +358,7 @@ void hro_mstvd_integrate(hro_ode *ode, hro_rhs_fn fu, void *ctx, double *u,
         if (hro_is_done(*t, tout, dt)) break;
         fu(ctx, *t, u, udot);
         const double *u4 = COL(uold, 4), *ud4 = COL(udotold, 4);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096)
         for (int64_t i = 0; i < n; ++i)
             ui[i] = ((((25 * u[i]) + (50 * dt) * udot[i]) + 7 * u4[i]) + (10 * dt) * ud4[i]) / 32;
         *t = *t + dt;
@@ -394,6 +402,7 @@ void hro_rhs_burgers(void *ctx, double t, const double *v, double *vdot)
     double alpha = c->alpha_fixed;
     if (c->flux == HRO_FLUX_LF && alpha < 0) {       /* alpha = maxval(abs(v)) */
         alpha = 0.0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (n > 4096) reduction(max : alpha)
         for (int i = 0; i < n; ++i) alpha = fabs(v[i]) > alpha ? fabs(v[i]) : alpha;
     }
     double *F = c->work;                              /* fedges(0:n) */
