@@ -377,8 +377,9 @@ def main() -> int:
                          "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards "
-              "by domain decomposition, which this build does not launch.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path would "
+              "shard by domain decomposition (lines of a batch: by line), which this library "
+              "does not launch.", file=sys.stderr)
         return 2
     if args.steps is None:
         args.steps = {"plain": 100, "batch": 200, "pbe": 500, "adjoint": 400,

@@ -843,7 +843,8 @@ static int run_stage_sets(const hrw_problem_s *p, const unsigned long long *ain,
         a.h = h;
         a.stage = stage;
         a.ty = p->ty;
-        a.nstrips = (p->n1 + kStrip - 1) / kStrip;
+        const int sw = strip_width(p->k, p->arith == HRW_ARITH_FAST);
+        a.nstrips = (p->n1 + sw - 1) / sw;
         launch_stage2d(p, a, s);
     }
     LAUNCH_CHECK();
@@ -1144,7 +1145,9 @@ extern "C" int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tou
 // stage covers every line (chunk table, per-line k/dt/grid/alpha), optionally replayed from a
 // captured hipGraph.  The plain path (one set of launches per line) is kept for comparison.
 struct hrw_batch_s {
-    int nlines = 0, numflux = HRW_FLUX_LAX_FRIEDRICHS, arith = HRW_ARITH_FAST, R = 8;
+    // cells per lane and kernel form until tuned: the fastest pair on the 4096-line ensemble of
+    // 256-65536 cells
+    int nlines = 0, numflux = HRW_FLUX_LAX_FRIEDRICHS, arith = HRW_ARITH_FAST, R = 6;
     int form = HRW_BATCH_CHUNK;
     double eps = 1e-6, alpha = -1.0;
     int64_t total = 0;

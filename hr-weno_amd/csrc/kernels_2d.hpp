@@ -8,7 +8,9 @@
 // along x2 (a window sliding down the column in registers) and along x1 (its window gathered
 // from the neighbouring lanes' column values by lane shifts, plus K-1 halo values loaded by
 // the edge lanes).  Interface fluxes along x1 pass between neighbouring lanes the same way, so
-// lanes 1..62 own output cells.  Each cell value is read from HBM once per row band.
+// lanes 1..62 own output cells.  (K = 3, FAST: columns X0-2 .. X0+61, lanes 2..61 own, and the
+// x1 smoothness triples are shared between lanes -- strip_width.)  Each cell value is read
+// from HBM once per row band.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,6 +33,11 @@ struct Stage2DArgs {
 };
 
 constexpr int kStrip = 62;
+// Output columns of a wavefront's strip.  K = 3 in the FAST form: 60, with two halo lanes each
+// side, so that every lane forms only the smoothness triple centred on its own column and takes
+// the triples of its neighbours' columns by lane shifts (the outer halo lanes' centred triples
+// feed lanes 1 and 62, whose reconstructions give the strip's two outer fluxes).
+__host__ __device__ constexpr int strip_width(int K, bool fast) { return (K == 3 && fast) ? 60 : kStrip; }
 
 template <int K, bool FAST>
 __device__ __forceinline__ void recon(const double *x, double eps, double &vl, double &vr)
@@ -164,14 +171,18 @@ __device__ __forceinline__ void pbe2d_wave(const Stage2DArgs &a)
     const long long Y0 = sy * a.ty;
     if (Y0 >= n2) return;                               // whole wavefront leaves together
     const int Y1 = (int)min((long long)n2, Y0 + a.ty);
-    const int i = sx * kStrip - 1 + lane;               // column of this lane (may be -1 or >= n1)
+    constexpr bool SHARE = K == 3 && FAST;             // centred triples shared between lanes
+    constexpr int S = strip_width(K, FAST), H0 = SHARE ? 2 : 1;   // output columns, left halo lanes
+    const int i = sx * S - H0 + lane;                   // column of this lane (may be < 0 or >= n1)
     auto colc = [n1](int c) { return c < 0 ? 0 : (c >= n1 ? n1 - 1 : c); };
     const int ic = colc(i);                             // clamped column for loads
-    const bool own = lane >= 1 && lane <= kStrip && i < n1;
-    // halo column: lanes 0..K-2 need i-(K-1), lanes 64-(K-1)..63 need i+(K-1); the other lanes
-    // read the nearest halo lane's column (no extra cache lines)
+    const bool own = lane >= H0 && lane < H0 + S && i < n1;
+    // halo column: lanes 0..K-2 need i-(K-1), lanes 64-(K-1)..63 need i+(K-1) (shared triples:
+    // lane 0 needs i-1 and lane 63 i+1); the other lanes read the nearest halo lane's column (no
+    // extra cache lines)
     const int hl = lane < 32 ? min(lane, K - 2) : max(lane, 64 - (K - 1));
-    const int hc = colc(sx * kStrip - 1 + hl + (lane < 32 ? -(K - 1) : K - 1));
+    const int hc = SHARE ? colc(sx * S - H0 + (lane < 32 ? -1 : 64))
+                         : colc(sx * S - H0 + hl + (lane < 32 ? -(K - 1) : K - 1));
 
     // per-lane constants: x1 width, g1 at the right interface x_{i+1/2}
     const double e_l = linear_edge(a.x1min, a.rx1, ic), e_r = linear_edge(a.x1min, a.rx1, ic + 1);
@@ -234,10 +245,24 @@ __device__ __forceinline__ void pbe2d_wave(const Stage2DArgs &a)
         }
 
         // x1: reconstruction of (i, j), flux at i+1/2 with the right neighbour's vl
-        double xw[2 * K - 1];
-        x1_window<K>(uc, o.halo, xw);
         double vl1, vr1;
-        recon<K, FAST>(xw, a.eps, vl1, vr1);
+        if constexpr (SHARE) {
+            // T(i-1) = (u(i-1), u(i), u(i+1)) here; T(i-2) and T(i) are the neighbours' own
+            const double xl = from_left(o.halo, uc), xr = from_right(o.halo, uc);
+            const Tri3 tm = tri3(xl, uc, xr);
+            Tri3 tl{}, tr{};
+            tl.bC = from_left(tm.bC, tm.bC);
+            tl.P2 = from_left(tm.P2, tm.P2);
+            tl.P1 = from_left(tm.P1, tm.P1);
+            tr.bA = from_right(tm.bA, tm.bA);
+            tr.P0 = from_right(tm.P0, tm.P0);
+            tr.Pm = from_right(tm.Pm, tm.Pm);
+            recon3(12.0 * a.eps, tl, tm, tr, vl1, vr1);
+        } else {
+            double xw[2 * K - 1];
+            x1_window<K>(uc, o.halo, xw);
+            recon<K, FAST>(xw, a.eps, vl1, vr1);
+        }
         const double vl1_right = from_right(vl1, vl1);
         double f1_r;
         if (i < 0) f1_r = 0.0;                           // zero flux at x1 = x1min

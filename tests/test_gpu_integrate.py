@@ -366,6 +366,21 @@ def test_pbe_fast_within_tolerance_and_mass(oracle, hh, dev):
     assert abs(mass(g) - mass(u0)) <= 1e-12 * mass(u0)
 
 
+@pytest.mark.parametrize("n1", [1, 2, 3, 59, 60, 61, 119, 120, 121])
+def test_pbe_fast_k3_strip_edges(oracle, hh, dev, n1):
+    """FAST, k = 3: strips of 60 output columns whose lanes take their neighbours' smoothness
+    triples by lane shifts; grids that end just before, on and after a strip boundary stay
+    within the FAST tolerance of the oracle."""
+    n2 = 9
+    u0, e1, e2, w1, w2, dt = pbe_case(n1, n2, seed=13)
+    ode = hh.MsTvd(hh.PBE2D(n1, n2, k=3, bc=3, arith="fast"))
+    u = _t(u0, dev)
+    t = ode.integrate(u, 0.0, 9.5 * dt, dt)
+    ou, ot, fe = oracle.pbe_ms(u0, e1, e2, k=3, bc=1, tout=9.5 * dt, dt=dt)
+    assert t == ot and ode.fevals == fe
+    assert_close(_h(u), ou)
+
+
 @pytest.mark.parametrize("n1,n2", [(1, 1), (2, 3), (5, 4), (63, 2), (125, 9)])
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_pbe_small_and_ragged_exact_bitwise(oracle, hh, dev, n1, n2, k):
