@@ -176,10 +176,10 @@ def run_plain(args, torch, hh, W, dev):
         raise RuntimeError("non-finite state after the timed steps")
     log(f"timed {args.steps} steps: {ms:.3f} ms")
     # the library's automatic choice: lines of 2^21 cells or more run the persistent form with 4
-    # cells per lane, 2^19..2^21 one chunk per wavefront with 6, shorter ones with 4
+    # cells per lane, 2^20..2^21 one chunk per wavefront with 6, shorter ones with 4
     automatic = args.variant == 0 and not args.cells_per_lane
     persistent = args.variant == 2 or (automatic and c.n >= (1 << 21))
-    cpl = args.cells_per_lane or (6 if not persistent and c.n >= (1 << 19) else 4)
+    cpl = args.cells_per_lane or (6 if not persistent and c.n >= (1 << 20) else 4)
     if args.arith != "fast" or args.variant == 1:
         kernel = "hrw::stage1d_kernel<3,LF,periodic,%s,alphamax,4>" % args.arith
     else:

@@ -751,12 +751,13 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
         const bool per = p->bc == HRW_BC_PERIODIC;
         // automatic (variant 0, cells_per_lane 0), measured best on 2^16..2^24 cells: lines of
         // 2^21 cells or more (several chunks per resident wavefront) the persistent form with 4
-        // cells per lane; 2^19..2^21 one chunk per wavefront with 6; shorter lines one chunk per
-        // wavefront with 4 (more wavefronts to hide the load latency).
+        // cells per lane; 2^20..2^21 one chunk per wavefront with 6; shorter lines one chunk per
+        // wavefront with 4 (more wavefronts to hide the load latency; with K = 3 the 4-cell
+        // kernel also shares its neighbours' triples).
         const bool automatic = p->variant == 0 && p->cells_per_lane == 0;
         const bool persistent = p->variant == 2 || (automatic && a.n >= (1LL << 21));
         const int cpl = p->cells_per_lane ? p->cells_per_lane
-                                          : (!persistent && a.n >= (1LL << 19) ? 6 : 4);
+                                          : (!persistent && a.n >= (1LL << 20) ? 6 : 4);
         const int form = persistent ? FORM_PERSISTENT : FORM_CHUNK;
         if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, form, cpl, a, s);
         else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, form, cpl, a, s);
