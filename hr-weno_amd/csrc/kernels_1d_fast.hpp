@@ -698,21 +698,45 @@ void batch_stage_kernel(BatchArgs b)
              l += (long long)gridDim.x * kBlock)
             b.alpha_clear[l] = 0ull;
     }
-    if (wave >= b.nchunks) return;   // whole wavefronts leave together
-    const int line = b.chunk_line[wave];
-    const Stage1DArgs a = batch_line_args(b, line);
-    double alpha = b.alpha;
-    if constexpr (ALPHAMAX) alpha = __longlong_as_double((long long)b.alpha_in[line]);
-    const int W = b.chunk_start[wave];
-    double m;
-    switch (b.k[line]) {
-    case 1: m = chunk_load_stage<1, FLUX, true, R, NPW>(a, W, alpha); break;
-    case 2: m = chunk_load_stage<2, FLUX, true, R, NPW>(a, W, alpha); break;
-    default: m = chunk_load_stage<3, FLUX, true, R, NPW>(a, W, alpha); break;
+    // (whole wavefronts past the table do no chunk, and take part in the block's barrier)
+    int line = -1;
+    double m = 0.0;
+    if (wave < b.nchunks) {
+        line = b.chunk_line[wave];
+        const Stage1DArgs a = batch_line_args(b, line);
+        double alpha = b.alpha;
+        if constexpr (ALPHAMAX) alpha = __longlong_as_double((long long)b.alpha_in[line]);
+        const int W = b.chunk_start[wave];
+        switch (b.k[line]) {
+        case 1: m = chunk_load_stage<1, FLUX, true, R, NPW>(a, W, alpha); break;
+        case 2: m = chunk_load_stage<2, FLUX, true, R, NPW>(a, W, alpha); break;
+        default: m = chunk_load_stage<3, FLUX, true, R, NPW>(a, W, alpha); break;
+        }
     }
     if constexpr (ALPHAMAX) {
+        // one atomic per line present in the block: the block's wavefronts hold consecutive
+        // chunks of the table, mostly of one line, and on a few long lines thousands of
+        // wavefronts would otherwise meet on one slot's cache line
+        __shared__ unsigned long long s_m[kBlock / 64];
+        __shared__ int s_line[kBlock / 64];
         m = wave_max(m);
-        if (lane == 0) atomicMax(b.alpha_out + line, (unsigned long long)__double_as_longlong(m));
+        const int wv = threadIdx.x >> 6;
+        if (lane == 0) {
+            s_m[wv] = (unsigned long long)__double_as_longlong(m);
+            s_line[wv] = line;
+        }
+        __syncthreads();
+        if (threadIdx.x < kBlock / 64) {
+            const int me = threadIdx.x, l = s_line[me];
+            bool first = l >= 0;
+            unsigned long long mx = s_m[me];
+#pragma unroll
+            for (int v = 0; v < kBlock / 64; ++v) {
+                if (v < me && s_line[v] == l) first = false;
+                if (v > me && s_line[v] == l) mx = s_m[v] > mx ? s_m[v] : mx;
+            }
+            if (first) atomicMax(b.alpha_out + l, mx);
+        }
     }
 }
 
