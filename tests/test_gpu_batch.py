@@ -130,3 +130,32 @@ def test_config3_subset(oracle, hh, dev):
                                            dt=c.dt)
         scale = np.abs(ou).max()
         assert np.max(np.abs(out[l] - ou)) <= TOL * scale
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_few_long_lines_match_per_line(hh, dev, form):
+    """A few long lines: whole blocks of wavefronts on one line, whose line maxima are reduced in
+    the block before the line's slot is updated; every alpha and so every bit as per line."""
+    import torch
+    from hrweno_hip import workloads as W
+    lens = np.array([65536, 70001, 131071])
+    ks = np.array([3, 2, 3])
+    u0s, dts = [], []
+    for l, n in enumerate(lens):
+        e, xc, w = W.linear_centres(-1.0, 1.0, int(n))
+        u0 = W.smooth_step_ic(xc, W.Rng(500 + l))
+        u0s.append(u0)
+        dts.append(W.cfl_dt(u0, 2.0 / n))
+    dts = np.array(dts)
+    outs = []
+    for per_line in (False, True):
+        b = hh.BurgersBatch(lens, ks, dts)
+        b.set_form(form)
+        u = b.pack(u0s, dev)
+        if per_line:
+            b.step_per_line(u, 4)
+        else:
+            b.step(u, 4, graph=True)
+        torch.cuda.synchronize()
+        outs.append(u)
+    assert torch.equal(outs[0], outs[1])
