@@ -669,6 +669,34 @@ struct BatchArgs {
     long long ninterior;                  // chunks [0, ninterior) are interior for K = 3
 };
 
+// The block's wavefront maxima into their lines' slots, one atomic per line present in the
+// block: the block's wavefronts hold consecutive chunks of the table, mostly of one line, and on
+// a few long lines thousands of wavefronts would otherwise meet on one slot's cache line.
+// line < 0: the wavefront has no chunk.  Every thread of the block calls it.
+__device__ __forceinline__ void block_line_max(unsigned long long *slots, int line, double m)
+{
+    __shared__ unsigned long long s_m[kBlock / 64];
+    __shared__ int s_line[kBlock / 64];
+    m = wave_max(m);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_m[wv] = (unsigned long long)__double_as_longlong(m);
+        s_line[wv] = line;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBlock / 64) {
+        const int me = threadIdx.x, l = s_line[me];
+        bool first = l >= 0;
+        unsigned long long mx = s_m[me];
+#pragma unroll
+        for (int v = 0; v < kBlock / 64; ++v) {
+            if (v < me && s_line[v] == l) first = false;
+            if (v > me && s_line[v] == l) mx = s_m[v] > mx ? s_m[v] : mx;
+        }
+        if (first) atomicMax(slots + l, mx);
+    }
+}
+
 // Line `line` of the batch as single-line stage arguments.
 __device__ __forceinline__ Stage1DArgs batch_line_args(const BatchArgs &b, int line)
 {
@@ -692,7 +720,6 @@ void batch_stage_kernel(BatchArgs b)
 {
     // wave-uniform, so the chunk table and the line's parameters come through scalar loads
     const long long wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)));
-    const int lane = threadIdx.x & 63;
     if (ALPHAMAX && b.alpha_clear) {
         for (long long l = (long long)blockIdx.x * kBlock + threadIdx.x; l < b.nlines;
              l += (long long)gridDim.x * kBlock)
@@ -713,31 +740,7 @@ void batch_stage_kernel(BatchArgs b)
         default: m = chunk_load_stage<3, FLUX, true, R, NPW>(a, W, alpha); break;
         }
     }
-    if constexpr (ALPHAMAX) {
-        // one atomic per line present in the block: the block's wavefronts hold consecutive
-        // chunks of the table, mostly of one line, and on a few long lines thousands of
-        // wavefronts would otherwise meet on one slot's cache line
-        __shared__ unsigned long long s_m[kBlock / 64];
-        __shared__ int s_line[kBlock / 64];
-        m = wave_max(m);
-        const int wv = threadIdx.x >> 6;
-        if (lane == 0) {
-            s_m[wv] = (unsigned long long)__double_as_longlong(m);
-            s_line[wv] = line;
-        }
-        __syncthreads();
-        if (threadIdx.x < kBlock / 64) {
-            const int me = threadIdx.x, l = s_line[me];
-            bool first = l >= 0;
-            unsigned long long mx = s_m[me];
-#pragma unroll
-            for (int v = 0; v < kBlock / 64; ++v) {
-                if (v < me && s_line[v] == l) first = false;
-                if (v > me && s_line[v] == l) mx = s_m[v] > mx ? s_m[v] : mx;
-            }
-            if (first) atomicMax(b.alpha_out + l, mx);
-        }
-    }
+    if constexpr (ALPHAMAX) block_line_max(b.alpha_out, line, m);
 }
 
 // Persistent form of the batch: as many wavefronts as fit on the device, wave w taking chunks
@@ -843,14 +846,15 @@ static __global__ __launch_bounds__(kBlock) void batch_maxabs_kernel(const doubl
 {
     const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wave >= nchunks) return;
-    const int line = chunk_line[wave];
-    const long long base = off[line] + chunk_start[wave];
-    const int len = min(kOutLanes * R, n[line] - chunk_start[wave]);
+    int line = -1;
     double m = 0.0;
-    for (int i = lane; i < len; i += 64) m = fmax(m, fabs(u[base + i]));
-    m = wave_max(m);
-    if (lane == 0) atomicMax(set + line, (unsigned long long)__double_as_longlong(m));
+    if (wave < nchunks) {   // (wavefronts past the table take part in the block's barrier)
+        line = chunk_line[wave];
+        const long long base = off[line] + chunk_start[wave];
+        const int len = min(kOutLanes * R, n[line] - chunk_start[wave]);
+        for (int i = lane; i < len; i += 64) m = fmax(m, fabs(u[base + i]));
+    }
+    block_line_max(set, line, m);
 }
 
 static __global__ void fill_u64_kernel(unsigned long long *p, long long count)
