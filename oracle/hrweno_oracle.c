@@ -427,6 +427,30 @@ void hro_rhs_burgers(void *ctx, double t, const double *v, double *vdot)
     for (int i = 0; i < n; ++i) vdot[i] = -((F[i + 1] - F[i]) / c->width[i]);   /* :107 */
 }
 
+/* Test hook of the 2D composition (see hrweno_oracle.h). */
+static int g_pbe_plant = 0;
+void hro_pbe_plant(int which) { g_pbe_plant = which; }
+
+/* Coordinate at which the growth rate of interior face i (1..n-1) is evaluated: the face
+ * itself, right(i) (example2:101,110); planted mistake 1 takes the centre of the upwind cell i. */
+static double pbe_face_x(const double *right, int i)
+{
+    if (g_pbe_plant == 1) return (right[i - 2] + right[i - 1]) / 2;   /* right[-1] = edges(0) */
+    return right[i - 1];
+}
+
+/* Outflow flux at the high end of a line of n cells: f(vr(n)) with the growth rate at the
+ * boundary face right(n); planted mistake 2 takes vl(n), mistake 3 the rate at the centre of
+ * cell n. */
+static double pbe_outflow(double ga, double gb, const double *right, int n, const double *vl,
+                          const double *vr)
+{
+    const double x = g_pbe_plant == 3 ? (right[n - 2] + right[n - 1]) / 2 : right[n - 1];
+    const double vm = g_pbe_plant == 2 ? vl[n - 1] : vr[n - 1];
+    const double g = ga + gb * x;
+    return hro_godunov(g * vm, g * vm, vm, vm);
+}
+
 /* example/example2_pbe_2d_fv.f90:73-129 */
 void hro_rhs_pbe2d(void *ctx, double t, const double *v, double *vdot)
 {
@@ -452,31 +476,21 @@ void hro_rhs_pbe2d(void *ctx, double t, const double *v, double *vdot)
             int64_t o = (int64_t)(j - 1) * n1;
             hro_weno_reconstruct(n1, c->k, c->eps, NULL, v + o, 1, vl, 1, vr, 1);
             for (int i = 1; i <= n1 - 1; ++i) {
-                double g = c->g1a + c->g1b * c->right1[i - 1];
+                double g = c->g1a + c->g1b * pbe_face_x(c->right1, i);
                 F1_(i, j) = hro_godunov(g * vr[i - 1], g * vl[i], vr[i - 1], vl[i]);
             }
             F1_(0, j) = 0.0;
-            if (c->bc == 0) {
-                F1_(n1, j) = 0.0;
-            } else {
-                double g = c->g1a + c->g1b * c->right1[n1 - 1];
-                F1_(n1, j) = hro_godunov(g * vr[n1 - 1], g * vr[n1 - 1], vr[n1 - 1], vr[n1 - 1]);
-            }
+            F1_(n1, j) = c->bc == 0 ? 0.0 : pbe_outflow(c->g1a, c->g1b, c->right1, n1, vl, vr);
         }
 #pragma omp for schedule(static)
         for (int i = 1; i <= n1; ++i) {           /* x2 lines, example2:106-112 */
             hro_weno_reconstruct(n2, c->k, c->eps, NULL, v + (i - 1), n1, vl, 1, vr, 1);
             for (int j = 1; j <= n2 - 1; ++j) {
-                double g = c->g2a + c->g2b * c->right2[j - 1];
+                double g = c->g2a + c->g2b * pbe_face_x(c->right2, j);
                 F2_(j, i) = hro_godunov(g * vr[j - 1], g * vl[j], vr[j - 1], vl[j]);
             }
             F2_(0, i) = 0.0;
-            if (c->bc == 0) {
-                F2_(n2, i) = 0.0;
-            } else {
-                double g = c->g2a + c->g2b * c->right2[n2 - 1];
-                F2_(n2, i) = hro_godunov(g * vr[n2 - 1], g * vr[n2 - 1], vr[n2 - 1], vr[n2 - 1]);
-            }
+            F2_(n2, i) = c->bc == 0 ? 0.0 : pbe_outflow(c->g2a, c->g2b, c->right2, n2, vl, vr);
         }
 #pragma omp for schedule(static)
         for (int j = 1; j <= n2; ++j)             /* example2:123-127 */
@@ -557,6 +571,32 @@ int hro_pbe_ms(int n1, int n2, int k, double eps, int bc, double g1a, double g1b
     hro_ode_free(&ode);
     free(c.work); free(w1); free(w2); free(ctr1); free(ctr2);
     return fe;
+}
+
+int hro_pbe_rhs(int n1, int n2, int k, double eps, int bc, double g1a, double g1b, double g2a,
+                double g2b, const double *edges1, const double *edges2, const double *v,
+                double *vdot)
+{
+    hro_pbe_ctx c;
+    c.n1 = n1; c.n2 = n2; c.k = k; c.bc = bc; c.eps = eps;
+    c.g1a = g1a; c.g1b = g1b; c.g2a = g2a; c.g2b = g2b;
+    double *w1 = (double *)malloc((size_t)n1 * sizeof(double));
+    double *w2 = (double *)malloc((size_t)n2 * sizeof(double));
+    double *ctr1 = (double *)malloc((size_t)n1 * sizeof(double));
+    double *ctr2 = (double *)malloc((size_t)n2 * sizeof(double));
+    const int nmax = n1 > n2 ? n1 : n2;
+    size_t wn = (size_t)2 * (n1 + 1) * (n2 + 1) + (size_t)2 * nmax * (g_threads + 1) + 16;
+    c.work = (double *)malloc(wn * sizeof(double));
+    if (!w1 || !w2 || !ctr1 || !ctr2 || !c.work) {
+        free(w1); free(w2); free(ctr1); free(ctr2); free(c.work);
+        return -2;
+    }
+    hro_grid_compute(n1, edges1, ctr1, w1);
+    hro_grid_compute(n2, edges2, ctr2, w2);
+    c.right1 = edges1 + 1; c.right2 = edges2 + 1; c.width1 = w1; c.width2 = w2;
+    hro_rhs_pbe2d(&c, 0.0, v, vdot);
+    free(c.work); free(w1); free(w2); free(ctr1); free(ctr2);
+    return 0;
 }
 
 int hro_linear_ode(int n, const double *a, int order, double *u, double *t,

@@ -122,6 +122,15 @@ int hro_burgers_ms(int n, int k, double eps, int flux, int bc, double alpha_fixe
 int hro_pbe_ms(int n1, int n2, int k, double eps, int bc, double g1a, double g1b,
                double g2a, double g2b, const double *edges1, const double *edges2,
                int order, double *u, double *t, double tout, double dt);
+/* One evaluation of hro_rhs_pbe2d on the linear grids edges1(0:n1), edges2(0:n2). */
+int hro_pbe_rhs(int n1, int n2, int k, double eps, int bc, double g1a, double g1b, double g2a,
+                double g2b, const double *edges1, const double *edges2, const double *v,
+                double *vdot);
+/* Test hook of hro_rhs_pbe2d: 0 = the example-2 composition; 1, 2, 3 plant one mistake each
+ * (interior growth rates at the upwind cell's centre instead of the face, the outflow flux from
+ * vl(n) instead of vr(n), the outflow growth rate at the centre of cell n instead of the
+ * boundary face) so tests can show the checks catch them. */
+void hro_pbe_plant(int which);
 /* Linear ODE with rktvd (order 1..3) or mstvd (order 0). */
 int hro_linear_ode(int n, const double *a, int order, double *u, double *t,
                    double tout, double dt);

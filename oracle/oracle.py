@@ -68,6 +68,8 @@ class Oracle:
         lib.hro_burgers_rk.restype = _L
         lib.hro_pbe_ms.argtypes = [_I, _I, _I, _D, _I, _D, _D, _D, _D, _P, _P, _I, _P, _P, _D,
                                    _D]
+        lib.hro_pbe_rhs.argtypes = [_I, _I, _I, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P]
+        lib.hro_pbe_plant.argtypes = [_I]
         lib.hro_burgers_ms.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P, _P, _D, _D]
         lib.hro_linear_ode.argtypes = [_I, _P, _I, _P, _P, _D, _D]
         lib.hro_burgers_rhs.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P]
@@ -211,6 +213,20 @@ class Oracle:
                                  g2[1], _ptr(e1), _ptr(e2), order, _ptr(u), C.byref(t), tout,
                                  dt)
         return u, t.value, fe
+
+    def pbe_rhs(self, v, edges1, edges2, *, k=3, eps=1e-6, bc=1, g1=(1.0, 0.5), g2=(0.5, 0.0)):
+        """One evaluation of the 2D PBE right-hand side (example 2's composition)."""
+        v = _f64(v)
+        e1, e2 = _f64(edges1), _f64(edges2)
+        out = np.empty_like(v)
+        if self.lib.hro_pbe_rhs(e1.size - 1, e2.size - 1, k, eps, bc, g1[0], g1[1], g2[0],
+                                g2[1], _ptr(e1), _ptr(e2), _ptr(v), _ptr(out)) != 0:
+            raise MemoryError("hro_pbe_rhs")
+        return out
+
+    def pbe_plant(self, which: int) -> None:
+        """Test hook: plant mistake `which` (1..3) in the 2D composition, 0 restores it."""
+        self.lib.hro_pbe_plant(int(which))
 
     def burgers_ms(self, u0, width, *, k=3, eps=1e-6, flux=FLUX_LF, bc=BC_PERIODIC, alpha=-1.0,
                    t0=0.0, tout=1.0, dt=1e-3, cnu=None):
