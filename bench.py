@@ -7,16 +7,28 @@ of each stage input, SSP-RK3 at CFL 0.4.  A "step" is one RK3 time step of the w
 three fused stage launches.  The state stays resident in HBM; only the integrate calls are
 timed (HIP events on the launch stream).
 
-Other workloads (--workload): `batch` -- 4096 independent periodic lines of log-uniform length
-in [256, 65536], k in {2, 3}, own dt, 200 RK3 steps, one launch per stage for all lines (graph
-replayed) vs one launch per line; `pbe` -- 2D population balance on 2048 x 2048 with the
-5-step multistep method, 500 steps.
+The other legs of the metric run in the same invocation, each at its own configuration and
+step count (whatever --steps says for the headline), and are reported as sub-objects of the one
+JSON line, each with its own `roofline` from this run's own timing:
 
-Prints ONE JSON line.  `roofline` prices the dominant kernel against HBM with the algorithmic
-bytes per cell-step (RK3 1D: 16 + 24 + 24 = 64; multistep 2D: 40); `cpu_baseline` times the
-reference's own compiled hrweno_weno/hrweno_fluxes (oracle/_ref) on a bounded sample on one
-core, and beside it (`all_cores`, labelled apart) the oracle's C restatement with OpenMP on every
-core the job may use, with the host's CPU model and core counts (`host`).
+  `batch`         -- 4096 independent periodic lines of log-uniform length in [256, 65536],
+                     k in {2, 3}, own dt, 200 RK3 steps, one launch per stage for all lines
+                     (autotuned, graph replayed) vs one launch per line (20 steps);
+  `adjoint`       -- gradient of 0.5*|u_N - target|^2 w.r.t. u0, 2^20 cells x 400 RK3 steps:
+                     the automatic tape (every stage input kept) and a checkpoint every 20 steps,
+                     each as gradient time / plain forward time, and the reverse kernel's roofline;
+  `adjoint_batch` -- the same over 256 lines of 1024..16384 cells vs one adjoint per line;
+  `pbe`           -- 2D population balance on 2048 x 2048, 5-step multistep method, 500 steps.
+
+`--workload X` runs one leg alone as the headline (`--legs none` drops the sub-objects).
+
+`roofline` prices the dominant kernel against HBM with the algorithmic bytes per cell-step
+(RK3 1D: 16 + 24 + 24 = 64; multistep 2D: 40; reverse RK3 stage: 88); `traffic` is the HBM
+bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/traffic.json).
+`cpu_baseline` times the reference's own compiled hrweno_weno/hrweno_fluxes (oracle/_ref) on a
+bounded sample on one core, and beside it (`all_cores`, labelled apart) the oracle's C
+restatement with OpenMP on every core the job may use, with the host's CPU model and core
+counts (`host`).
 """
 from __future__ import annotations
 
@@ -32,6 +44,18 @@ sys.path.insert(0, os.path.join(REPO, "hr-weno_amd"))
 HBM_PEAK_GBS = 8000.0
 METRIC = "FP64 cell-updates/s & % HBM peak: plain vs batched vs adjoint, ≤1e-12 rel"
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.json")
+
+
+# steps of each workload when it is not the headline (the configurations' own horizons)
+LEG_STEPS = {"plain": 100, "batch": 200, "pbe": 500, "adjoint": 400, "adjoint_batch": 400}
+
+
+def release(torch) -> None:
+    """Return the device memory of the leg just run before the next one allocates."""
+    import gc
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
 
 
 def log(msg: str) -> None:
@@ -236,20 +260,34 @@ def run_batch(args, torch, hh, W, dev):
 
 
 def run_pbe(args, torch, hh, W, dev):
+    """The multistep method amplifies rounding on this problem (a 1-ulp change of u0 reaches
+    ~1e-1 by step 500 in the reference's own arithmetic), so over its 500 steps only the EXACT
+    form (the reference's evaluation order, bit-identical) meets 1e-12; that is the timed one.
+    The FAST form (within 1e-12 for ~200 steps) is timed beside it for comparison."""
     c = W.config4(steps=args.steps)
-    ode = hh.MsTvd(hh.PBE2D(c.n1, c.n2, k=3, bc=hh.BC_INFLOW0_OUTFLOW, arith=args.arith))
-    u = torch.as_tensor(c.u0, device=dev).clone()
-    t = ode.integrate(u, 0.0, W.steps_tout(0.0, c.dt, 4 + args.warmup), c.dt)   # startup + warmup
-    tout = W.steps_tout(t, c.dt, args.steps)
-    fe0 = ode.fevals
-    ms = timed(torch, lambda: ode.integrate(u, t, tout, c.dt))
-    assert ode.fevals - fe0 == args.steps
     ncell = c.n1 * c.n2
+    arith = "exact" if args.pbe_arith == "auto" else args.pbe_arith
+    out = {}
+    for form in dict.fromkeys([arith, "fast"]):
+        ode = hh.MsTvd(hh.PBE2D(c.n1, c.n2, k=3, bc=hh.BC_INFLOW0_OUTFLOW, arith=form))
+        u = torch.as_tensor(c.u0, device=dev).clone()
+        t = ode.integrate(u, 0.0, W.steps_tout(0.0, c.dt, 4 + args.warmup), c.dt)   # startup + warmup
+        tout = W.steps_tout(t, c.dt, args.steps)
+        fe0 = ode.fevals
+        ms = timed(torch, lambda: ode.integrate(u, t, tout, c.dt))
+        assert ode.fevals - fe0 == args.steps
+        out[form] = ms
+        log(f"pbe {form}: {args.steps} steps {ms:.3f} ms")
+    ms = out[arith]
     rl = roofline(40.0 * ncell, ms / args.steps,
-                  "hrw::pbe2d_kernel<3,inflow0_outflow,%s>" % args.arith, "pbe2d_2048")
+                  "hrw::pbe2d_kernel<3,inflow0_outflow,%s>" % arith, "pbe2d_2048_" + arith)
     config = {"workload": "pbe2d_2048x2048_weno5_godunov_mstvd", "cells": ncell,
               "integrator": "mstvd (5-step, 3rd order)", "g1": "1+0.5*x1", "g2": "0.5",
-              "bc": "zero inflow, upwind outflow", "arith": args.arith,
+              "bc": "zero inflow, upwind outflow", "arith": arith,
+              "parity": "exact: bit-identical to the reference arithmetic over all 500 steps"
+                        if arith == "exact" else "fast: within 1e-12 for ~200 steps only",
+              "fast_cell_updates_per_s": ncell * args.steps / (out["fast"] / 1e3),
+              "fast_roofline_frac": roofline(40.0 * ncell, out["fast"] / args.steps, "", "")["frac"],
               "parallelism": "single GPU"}
     return ncell * args.steps / (ms / 1e3), ms / args.steps, rl, config, \
         "synthetic (seeded Gaussian pulse)"
@@ -364,6 +402,8 @@ def main() -> int:
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="adjoint checkpoint interval (0: the largest that fits in half of HBM)")
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--pbe-arith", choices=["auto", "fast", "exact"], default="auto",
+                    help="arithmetic form of the timed 2D workload (auto: exact)")
     ap.add_argument("--cells", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -371,7 +411,9 @@ def main() -> int:
                     help="1D FAST kernel: 0 register-blocked, 1 LDS tile")
     ap.add_argument("--cells-per-lane", type=int, default=0,
                     help="cells per lane of the 1D stage kernel (0: chosen by the line length)")
-    ap.add_argument("--per-line-steps", type=int, default=5)
+    ap.add_argument("--per-line-steps", type=int, default=20)
+    ap.add_argument("--legs", default="batch,adjoint,adjoint_batch,pbe",
+                    help="legs reported beside the headline ('none': the headline only)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of --cells doubles (20 times), the "
                          "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters")
@@ -382,8 +424,7 @@ def main() -> int:
               "does not launch.", file=sys.stderr)
         return 2
     if args.steps is None:
-        args.steps = {"plain": 100, "batch": 200, "pbe": 500, "adjoint": 400,
-                      "adjoint_batch": 400}[args.workload]
+        args.steps = LEG_STEPS[args.workload]
 
     import torch
     import hrweno_hip as hh
@@ -399,9 +440,10 @@ def main() -> int:
         log(f"copy probe: 20 copies of {args.cells * 8} bytes")
         return 0
 
-    runner = {"plain": run_plain, "batch": run_batch, "pbe": run_pbe, "adjoint": run_adjoint,
-              "adjoint_batch": run_adjoint_batch}[args.workload]
-    value, ms_step, rl, config, data = runner(args, torch, hh, W, dev)
+    runners = {"plain": run_plain, "batch": run_batch, "pbe": run_pbe, "adjoint": run_adjoint,
+               "adjoint_batch": run_adjoint_batch}
+    value, ms_step, rl, config, data = runners[args.workload](args, torch, hh, W, dev)
+    release(torch)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "plain":
@@ -414,6 +456,21 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": data, "config": config, "roofline": rl, "cpu_baseline": cpu,
     }
+    legs = [] if args.legs in ("", "none") else [x.strip() for x in args.legs.split(",")]
+    for leg in legs:
+        if leg == args.workload:
+            continue
+        if leg not in runners:
+            raise SystemExit(f"unknown leg {leg!r}")
+        sub = argparse.Namespace(**vars(args))
+        sub.workload = leg
+        sub.steps = LEG_STEPS[leg]
+        sub.ckpt_every = 0
+        log(f"leg {leg}: {sub.steps} steps")
+        v, ms, lrl, lconf, ldata = runners[leg](sub, torch, hh, W, dev)
+        release(torch)
+        out[leg] = {"value": v, "unit": "cell-updates/s", "steps": sub.steps,
+                    "ms_per_step": ms, "config": lconf, "roofline": lrl, "data": ldata}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -115,6 +115,14 @@ extern "C" int hrw_device_alloc(size_t bytes, void **ptr)
         return fail(HRW_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     return HRW_OK;
 }
+extern "C" int hrw_device_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return HRW_OK;
+}
 extern "C" int hrw_device_free(void *ptr)
 {
     HIP_TRY(hipFree(ptr));
@@ -522,6 +530,58 @@ extern "C" int hrw_reconstruct_dev(int n, int k, double eps, int arith, const do
     return reconstruct_core(a, S(stream));
 }
 
+// Elements addressed by nlines lines of n elements (element i of line l at l*ld + i*inc).
+static int64_t lines_span(int64_t n, int64_t nlines, int64_t inc, int64_t ld)
+{
+    return (nlines - 1) * ld + (n - 1) * inc + 1;
+}
+
+static int check_lines(int64_t n, int64_t nlines, int64_t incv, int64_t ldv, int64_t inco,
+                       int64_t ldo)
+{
+    if (nlines < 1 || nlines > 65535) return fail(HRW_ERR_ARG, "nlines must be in [1, 65535]");
+    if (incv < 1 || inco < 1 || ldv < 0 || ldo < 0)
+        return fail(HRW_ERR_ARG, "line strides must be positive (inc >= 1, ld >= 0)");
+    (void)n;
+    return HRW_OK;
+}
+
+extern "C" int hrw_reconstruct_lines_dev(int ncells, int k, double eps, int arith, int64_t nlines,
+                                         const double *v, int64_t incv, int64_t ldv, double *vl,
+                                         double *vr, int64_t inco, int64_t ldo,
+                                         hrw_stream_t stream)
+{
+    CALL(check_weno_args(ncells, k, eps, arith));
+    CALL(check_lines(ncells, nlines, incv, ldv, inco, ldo));
+    ReconArgs a{ncells, k, arith, eps, nullptr, nlines, v, incv, ldv, vl, inco, ldo, vr, inco, ldo};
+    return reconstruct_core(a, S(stream));
+}
+
+extern "C" int hrw_reconstruct_lines_host(int ncells, int k, double eps, int arith,
+                                          const double *cnu, int64_t nlines, const double *v,
+                                          int64_t incv, int64_t ldv, double *vl, double *vr,
+                                          int64_t inco, int64_t ldo)
+{
+    CALL(check_weno_args(ncells, k, eps, arith));
+    CALL(check_lines(ncells, nlines, incv, ldv, inco, ldo));
+    if (!v || !vl || !vr) return fail(HRW_ERR_ARG, "null array");
+    const int64_t sv = lines_span(ncells, nlines, incv, ldv);
+    const int64_t so = lines_span(ncells, nlines, inco, ldo);
+    const size_t ncnu = cnu ? (size_t)ncells * k * (k + 1) : 0;
+    CALL(g_stage.reserve((size_t)sv + 2 * (size_t)so + ncnu));
+    double *dv = g_stage.buf, *dl = dv + sv, *dr = dl + so, *dc = cnu ? dr + so : nullptr;
+    HIP_TRY(hipMemcpy(dv, v, (size_t)sv * sizeof(double), hipMemcpyHostToDevice));
+    if (cnu) HIP_TRY(hipMemcpy(dc, cnu, ncnu * sizeof(double), hipMemcpyHostToDevice));
+    // the outputs' untouched elements (gaps between strided lines) keep the caller's values
+    HIP_TRY(hipMemcpy(dl, vl, (size_t)so * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr, vr, (size_t)so * sizeof(double), hipMemcpyHostToDevice));
+    ReconArgs a{ncells, k, arith, eps, dc, nlines, dv, incv, ldv, dl, inco, ldo, dr, inco, ldo};
+    CALL(reconstruct_core(a, nullptr));
+    HIP_TRY(hipMemcpy(vl, dl, (size_t)so * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vr, dr, (size_t)so * sizeof(double), hipMemcpyDeviceToHost));
+    return HRW_OK;
+}
+
 extern "C" int hrw_weno_reconstruct_host(hrw_weno_t w, const double *v, int64_t incv,
                                          double *vl, double *vr)
 {
@@ -735,6 +795,38 @@ extern "C" int hrw_problem_neq(hrw_problem_t p, int64_t *neq)
 // ---- kernel dispatch ----------------------------------------------------------------
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
+// Every edge xmin + rx*i (i = 0..n) of a linear grid is computed without rounding, so every
+// width e(i+1) - e(i) (src/hrweno_grids.f90:247) is exactly rx: with rx = Mr*2^er and
+// xmin = Mx*2^ex (Mr, Mx odd integers), rx*i is exact while Mr*n < 2^53, and xmin + rx*i, a
+// multiple of 2^min(er, ex), is exact while its magnitude stays below 2^(53 + min(er, ex)).
+// (Grids of 2^m cells on [-1, 1] or [0, 1], for instance.)
+static bool exact_linear_grid(double xmin, double rx, long long n)
+{
+    if (!(rx > 0.0) || !std::isfinite(rx) || !std::isfinite(xmin) || n < 1) return false;
+    auto odd_significand = [](double x, int &e) {   // |x| = M * 2^e, M odd
+        int ex = 0;
+        const double m = std::frexp(std::fabs(x), &ex);
+        unsigned long long M = (unsigned long long)std::ldexp(m, 53);
+        e = ex - 53;
+        const int tz = __builtin_ctzll(M);
+        M >>= tz;
+        e += tz;
+        return M;
+    };
+    int er = 0;
+    const unsigned long long Mr = odd_significand(rx, er);
+    if ((long double)Mr * (long double)n >= 9007199254740992.0L) return false;
+    int e = er;
+    if (xmin != 0.0) {
+        int ex = 0;
+        (void)odd_significand(xmin, ex);
+        e = std::min(e, ex);
+    }
+    const long double big = std::max(std::fabs((long double)xmin),
+                                     std::fabs((long double)xmin + (long double)rx * (long double)n));
+    return big < std::ldexp(1.0L, 53 + e);
+}
+
 static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStream_t s)
 {
     const bool amax0 = p->alphamax();
@@ -759,9 +851,11 @@ static void launch_stage1d(const hrw_problem_s *p, const Stage1DArgs &a, hipStre
         const int cpl = p->cells_per_lane ? p->cells_per_lane
                                           : (!persistent && a.n >= (1LL << 20) ? 6 : 4);
         const int form = persistent ? FORM_PERSISTENT : FORM_CHUNK;
-        if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, form, cpl, a, s);
-        else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, form, cpl, a, s);
-        else launch_fast_1d_k3(p->numflux, per, amax0, form, cpl, a, s);
+        // every width exactly rx: the kernels take 1/width once per chunk (the same value)
+        const bool constw = exact_linear_grid(a.xmin, a.rx, a.n);
+        if (p->k == 1) launch_fast_1d_k1(p->numflux, per, amax0, constw, form, cpl, a, s);
+        else if (p->k == 2) launch_fast_1d_k2(p->numflux, per, amax0, constw, form, cpl, a, s);
+        else launch_fast_1d_k3(p->numflux, per, amax0, constw, form, cpl, a, s);
         return;
     }
     const bool per = p->bc == HRW_BC_PERIODIC, fast = p->arith == HRW_ARITH_FAST;
@@ -893,6 +987,8 @@ struct hrw_ode_s {
     unsigned long long *slots = nullptr;
     int tape_every = 0;                    // > 0: integrate records for hrw_ode_adjoint
     hrw_adjoint_s *adj = nullptr;
+    bool check_finite = true;              // integrate ends with a scan of u for NaN/Inf
+    unsigned long long *d_bad = nullptr;   // the scan's result word
     ~hrw_ode_s();
 };
 static int ode_free_adjoint(hrw_ode_s *o);
@@ -905,6 +1001,7 @@ hrw_ode_s::~hrw_ode_s()
     for (auto &b : ring_u) dfree(b);
     for (auto &b : ring_L) dfree(b);
     dfree(slots);
+    dfree(d_bad);
     ode_free_adjoint(this);
 }
 
@@ -919,6 +1016,7 @@ static int ode_alloc(hrw_ode_s *o)
         for (auto &b : o->ring_L) CALL(dalloc(&b, o->neq));
     }
     CALL(dalloc(&o->slots, (size_t)3 * kNSlot * kSlotStride));
+    CALL(dalloc(&o->d_bad, 1));
     return HRW_OK;
 }
 
@@ -1126,6 +1224,32 @@ static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
     return HRW_OK;
 }
 
+// NaN or Inf anywhere in u: one read of the state, one word back to the host.  The
+// reference has no such test (its fatal-error path, src/hrweno_tvdode.f90:286-297, covers bad
+// arguments); a blown-up state would otherwise integrate on silently.
+static __global__ __launch_bounds__(kBlock) void nonfinite_kernel(const double *__restrict__ u,
+                                                                 long long n,
+                                                                 unsigned long long *bad)
+{
+    bool b = false;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock)
+        b |= !isfinite(u[i]);
+    if (__any(b) && (threadIdx.x & 63) == 0) atomicOr(bad, 1ull);
+}
+
+static int ode_check_finite(hrw_ode_s *o, const double *u)
+{
+    fill_u64_kernel<<<1, 64, 0, o->stream>>>(o->d_bad, 1);
+    nonfinite_kernel<<<grid_for(o->neq, kBlock * 8, 4096), kBlock, 0, o->stream>>>(u, o->neq, o->d_bad);
+    LAUNCH_CHECK();
+    unsigned long long bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, o->d_bad, sizeof bad, hipMemcpyDeviceToHost, o->stream));
+    HIP_TRY(hipStreamSynchronize(o->stream));
+    if (bad) return fail(HRW_ERR_STATE, "non-finite state: u holds NaN or Inf after integrate");
+    return HRW_OK;
+}
+
 extern "C" int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tout, double dt,
                                  int itask)
 {
@@ -1135,10 +1259,118 @@ extern "C" int hrw_ode_integrate(hrw_ode_t ode, double *u, double *t, double tou
     // the reference loops forever on dt == 0 (is_done never turns true); refuse it instead
     if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(*t) || std::isnan(tout))
         return fail(HRW_ERR_ARG, "Invalid input 'dt', 't' or 'tout': dt must be finite and nonzero");
+    const double t_in = *t;
     int rc = ode->method == METHOD_MS ? ms_integrate(ode, u, t, tout, dt)
                                       : rk_integrate(ode, u, t, tout, dt, itask);
+    if (rc == HRW_OK && ode->check_finite && *t != t_in) rc = ode_check_finite(ode, u);
     if (rc != HRW_OK) ode->istate = -1;
     return rc;
+}
+
+extern "C" int hrw_ode_set_check(hrw_ode_t ode, int check_finite)
+{
+    if (!ode) return fail(HRW_ERR_ARG, "null ode");
+    ode->check_finite = check_finite != 0;
+    return HRW_OK;
+}
+
+// ---- serialised state (resume in another object or process) ---------------------------
+// Layout: StateHeader, then u (neq doubles), then for the multistep method the history
+// columns 1..4 (newest first) of u and of L(u), each neq doubles.
+namespace {
+struct StateHeader {
+    char magic[8];             // "HRWODE1\0"
+    int32_t method, order;
+    int64_t neq;
+    int32_t fevals, istate;
+    double t;
+    int64_t history;           // 1: the multistep history follows u
+};
+constexpr char kStateMagic[8] = {'H', 'R', 'W', 'O', 'D', 'E', '1', '\0'};
+}
+
+static int64_t ode_state_bytes(const hrw_ode_s *o)
+{
+    const int64_t cols = 1 + (o->method == METHOD_MS ? 8 : 0);
+    return (int64_t)sizeof(StateHeader) + cols * o->neq * (int64_t)sizeof(double);
+}
+
+extern "C" int hrw_ode_state_size(hrw_ode_t ode, int64_t *bytes)
+{
+    if (!ode || !bytes) return fail(HRW_ERR_ARG, "null argument");
+    *bytes = ode_state_bytes(ode);
+    return HRW_OK;
+}
+
+extern "C" int hrw_ode_save(hrw_ode_t ode, const double *u, double t, void *buf, int64_t bytes)
+{
+    if (!ode || !u || !buf) return fail(HRW_ERR_ARG, "null argument");
+    if (bytes < ode_state_bytes(ode))
+        return fail(HRW_ERR_ARG, "state buffer holds %lld bytes, %lld needed", (long long)bytes,
+                    (long long)ode_state_bytes(ode));
+    StateHeader h{};
+    std::memcpy(h.magic, kStateMagic, sizeof h.magic);
+    h.method = ode->method;
+    h.order = ode->order;
+    h.neq = ode->neq;
+    h.fevals = ode->fevals;
+    h.istate = ode->istate;
+    h.t = t;
+    h.history = ode->method == METHOD_MS && ode->istate >= 2 ? 1 : 0;
+    char *p = static_cast<char *>(buf);
+    std::memcpy(p, &h, sizeof h);
+    const size_t col = (size_t)ode->neq * sizeof(double);
+    double *dst = reinterpret_cast<double *>(p + sizeof h);
+    HIP_TRY(hipMemcpyAsync(dst, u, col, hipMemcpyDeviceToHost, ode->stream));
+    if (ode->method == METHOD_MS) {
+        for (int c = 1; c <= 4; ++c) {   // column c (1 newest .. 4 oldest) is ring index head+4-c
+            const int r = (ode->head + 4 - c) % 4;
+            if (h.history) {
+                HIP_TRY(hipMemcpyAsync(dst + (size_t)c * ode->neq, ode->ring_u[r], col,
+                                       hipMemcpyDeviceToHost, ode->stream));
+                HIP_TRY(hipMemcpyAsync(dst + (size_t)(4 + c) * ode->neq, ode->ring_L[r], col,
+                                       hipMemcpyDeviceToHost, ode->stream));
+            } else {
+                std::memset(dst + (size_t)c * ode->neq, 0, col);
+                std::memset(dst + (size_t)(4 + c) * ode->neq, 0, col);
+            }
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(ode->stream));
+    return HRW_OK;
+}
+
+extern "C" int hrw_ode_load(hrw_ode_t ode, double *u, double *t, const void *buf, int64_t bytes)
+{
+    if (!ode || !u || !t || !buf) return fail(HRW_ERR_ARG, "null argument");
+    if (bytes < (int64_t)sizeof(StateHeader)) return fail(HRW_ERR_ARG, "state buffer too short");
+    StateHeader h;
+    const char *p = static_cast<const char *>(buf);
+    std::memcpy(&h, p, sizeof h);
+    if (std::memcmp(h.magic, kStateMagic, sizeof h.magic) != 0)
+        return fail(HRW_ERR_ARG, "not a saved ODE state");
+    if (h.method != ode->method || h.order != ode->order || h.neq != ode->neq)
+        return fail(HRW_ERR_ARG, "saved state of a different ODE (method %d order %d neq %lld)",
+                    h.method, h.order, (long long)h.neq);
+    if (bytes < ode_state_bytes(ode)) return fail(HRW_ERR_ARG, "state buffer too short");
+    const size_t col = (size_t)ode->neq * sizeof(double);
+    const double *src = reinterpret_cast<const double *>(p + sizeof h);
+    HIP_TRY(hipMemcpyAsync(u, src, col, hipMemcpyHostToDevice, ode->stream));
+    if (ode->method == METHOD_MS && h.history) {
+        ode->head = 0;   // column c at ring index 4 - c
+        for (int c = 1; c <= 4; ++c) {
+            HIP_TRY(hipMemcpyAsync(ode->ring_u[(4 - c) % 4], src + (size_t)c * ode->neq, col,
+                                   hipMemcpyHostToDevice, ode->stream));
+            HIP_TRY(hipMemcpyAsync(ode->ring_L[(4 - c) % 4], src + (size_t)(4 + c) * ode->neq, col,
+                                   hipMemcpyHostToDevice, ode->stream));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(ode->stream));
+    ode->fevals = h.fevals;
+    // a multistep state saved before its history existed starts over (istate 1)
+    ode->istate = ode->method == METHOD_MS && !h.history && h.istate >= 2 ? 1 : h.istate;
+    *t = h.t;
+    return HRW_OK;
 }
 
 // ------------------------------------------------------------------ batched lines -----
@@ -1165,6 +1397,7 @@ struct hrw_batch_s {
     unsigned long long *line_slots = nullptr;  // per-line path: nlines x 3 x kNSlot x kSlotStride
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;   // ordering with the NULL stream
     hipGraphExec_t graph = nullptr;
     const double *graph_u = nullptr;
     int graph_R = 0, graph_form = 0;
@@ -1175,7 +1408,27 @@ struct hrw_batch_s {
         dfree(d_off); dfree(d_n); dfree(d_k); dfree(d_cline); dfree(d_cstart);
         dfree(d_h); dfree(d_xmin); dfree(d_rx); dfree(ui); dfree(ui2); dfree(slots);
         dfree(line_slots);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// A batch runs on a non-default stream (graph capture needs one).  When the caller gave none
+// (hrw_batch_set_stream), each call is ordered after the work already queued on the NULL
+// stream and the NULL stream after the call's work, so C and Fortran callers that use the
+// NULL stream (copies, their own kernels) see the batch's results in stream order.
+struct BatchCall {
+    hrw_batch_s *b;
+    explicit BatchCall(hrw_batch_s *bb) : b(bb)
+    {
+        if (b->own_stream && hipEventRecord(b->ev_in, nullptr) == hipSuccess)
+            (void)hipStreamWaitEvent(b->stream, b->ev_in, 0);
+    }
+    ~BatchCall()
+    {
+        if (b->own_stream && hipEventRecord(b->ev_out, b->stream) == hipSuccess)
+            (void)hipStreamWaitEvent(nullptr, b->ev_out, 0);
     }
 };
 
@@ -1273,6 +1526,8 @@ extern "C" int hrw_batch_burgers_create(int64_t nlines, const int *ncells, const
     CALL(batch_chunks(b, b->R));
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->own_stream = true;
+    HIP_TRY(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
     *out = guard.release();
     return HRW_OK;
 }
@@ -1288,6 +1543,13 @@ extern "C" int hrw_batch_size(hrw_batch_t b, int64_t *total, int64_t *offsets)
     if (!b) return fail(HRW_ERR_ARG, "null batch");
     if (total) *total = b->total;
     if (offsets) std::memcpy(offsets, b->h_off.data(), b->h_off.size() * sizeof(int64_t));
+    return HRW_OK;
+}
+
+extern "C" int hrw_batch_synchronize(hrw_batch_t b)
+{
+    if (!b) return fail(HRW_ERR_ARG, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
     return HRW_OK;
 }
 
@@ -1376,6 +1638,7 @@ extern "C" int hrw_batch_step(hrw_batch_t b, double *u, int64_t nsteps, double *
     if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
     if (((uintptr_t)u & 15u) != 0) return fail(HRW_ERR_ARG, "the batch state must be 16-byte aligned");
     if (nsteps == 0) return HRW_OK;
+    BatchCall order(b);
     CALL(batch_seed(b, u));
     if (use_graph) {
         if (!b->graph || b->graph_u != u || b->graph_R != b->chunk_R || b->graph_form != b->form) {
@@ -1414,6 +1677,7 @@ extern "C" int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps,
     if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
     const size_t per = (size_t)3 * kNSlot * kSlotStride;
     if (!b->line_slots) CALL(dalloc(&b->line_slots, per * b->nlines));
+    BatchCall order(b);
     hrw_problem_s p;
     p.kind = PROB_BURGERS1D;
     p.eps = b->eps;
@@ -1451,8 +1715,10 @@ extern "C" int hrw_batch_step_per_line(hrw_batch_t b, double *u, int64_t nsteps,
 extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
 {
     if (!b) return fail(HRW_ERR_ARG, "null batch");
-    double *scratch = nullptr;
-    CALL(dalloc(&scratch, (size_t)b->total));
+    if (!u) return fail(HRW_ERR_ARG, "null state");
+    BatchCall order(b);
+    double *work = nullptr;
+    CALL(dalloc(&work, (size_t)b->total));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1463,18 +1729,18 @@ extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
         b->form = cand / 3;
         int rc = batch_chunks(b, R);
         if (rc == HRW_OK) {
-            (void)hipMemcpyAsync(scratch, u, b->total * sizeof(double), hipMemcpyDeviceToDevice, b->stream);
-            rc = batch_seed(b, scratch);
-            if (rc == HRW_OK) rc = batch_rk3_step(b, scratch);   // warm-up
+            (void)hipMemcpyAsync(work, u, b->total * sizeof(double), hipMemcpyDeviceToDevice, b->stream);
+            rc = batch_seed(b, work);
+            if (rc == HRW_OK) rc = batch_rk3_step(b, work);   // warm-up
             (void)hipEventRecord(e0, b->stream);
-            for (int i = 0; rc == HRW_OK && i < 3; ++i) rc = batch_rk3_step(b, scratch);
+            for (int i = 0; rc == HRW_OK && i < 3; ++i) rc = batch_rk3_step(b, work);
             (void)hipEventRecord(e1, b->stream);
             (void)hipEventSynchronize(e1);
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (rc != HRW_OK) {
-            dfree(scratch);
+            dfree(work);
             return rc;
         }
         if (ms < best_ms) {
@@ -1485,7 +1751,7 @@ extern "C" int hrw_batch_tune(hrw_batch_t b, const double *u, int *chosen)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    dfree(scratch);
+    dfree(work);
     b->R = best;
     b->form = best_form;
     CALL(batch_chunks(b, best));
@@ -1670,6 +1936,7 @@ static int adjoint_chunks(hrw_adjoint_s *a, const std::vector<int> &ncells)
             c.xmin = a->p->xmin;
             c.rx = a->p->rx;
         }
+        c.constw = exact_linear_grid(c.xmin, c.rx, c.n) ? 1.0 : 0.0;
         for (long long st = 0; st < ncells[l]; st += 62LL * a->R) {
             c.start = (int)st;
             ch.push_back(c);
@@ -1893,6 +2160,7 @@ extern "C" int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN
     if (!u0 || !uN) return fail(HRW_ERR_ARG, "null state");
     const size_t bytes = (size_t)a->flat * sizeof(double);
     hipStream_t s = a->strm();
+    std::unique_ptr<BatchCall> order(a->b ? new BatchCall(a->b) : nullptr);
     HIP_TRY(hipMemcpyAsync(a->ckpt[0], u0, bytes, hipMemcpyDeviceToDevice, s));
     a->recorded = false;
     for (int64_t g = 0; g < a->nseg; ++g) CALL(adjoint_segment(a, g, true));
@@ -1975,6 +2243,7 @@ extern "C" int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g
     if (((uintptr_t)g0 & 15u) != 0) return fail(HRW_ERR_ARG, "the cotangent must be 16-byte aligned");
     const size_t bytes = (size_t)a->flat * sizeof(double);
     hipStream_t st = a->strm();
+    std::unique_ptr<BatchCall> order(a->b ? new BatchCall(a->b) : nullptr);
     double *lam = g0, *A = a->lamA, *B = a->lamB;
     a->owed.clear();
     if (gN != g0) HIP_TRY(hipMemcpyAsync(g0, gN, bytes, hipMemcpyDeviceToDevice, st));

@@ -69,7 +69,10 @@ __device__ __forceinline__ Tri3 tri3(double a, double b, double c)
 
 // One reconstruction from the pieces of T(j-2), T(j-1), T(j).  eps12 = 12*eps.  Each face
 // takes one reciprocal of its own sum of weights, so the only products are pairs of q's: finite
-// while beta < ~1e77, i.e. for cell-value jumps below ~1e38.
+// while beta < ~1e77, i.e. for cell-value jumps below ~1e38.  SCALED: vl and vr come out
+// multiplied by 6 (the candidates are 6 c(:,r).T; the caller's flux constants absorb the
+// factor, two multiplies fewer per cell).
+template <bool SCALED = false>
 __device__ __forceinline__ void recon3(double eps12, const Tri3 &t2, const Tri3 &t1,
                                        const Tri3 &t0, double &vl, double &vr)
 {
@@ -80,7 +83,11 @@ __device__ __forceinline__ void recon3(double eps12, const Tri3 &t2, const Tri3 
     const double m0 = q1 * q2, m1 = q0 * q2, m2 = q0 * q1;
     // d = (0.3, 0.6, 0.1) ~ (3, 6, 1) on the right face, reversed on the left face
     const double a0 = 3.0 * m0, a1 = 6.0 * m1, l2 = 3.0 * m2;
-    const double sr = 6.0 * (a0 + a1 + m2), sl = 6.0 * (m0 + a1 + l2);
+    double sr = a0 + a1 + m2, sl = m0 + a1 + l2;
+    if constexpr (!SCALED) {
+        sr = 6.0 * sr;
+        sl = 6.0 * sl;
+    }
     const double nr = fma_(a0, t0.P0, fma_(a1, t1.P1, m2 * t2.P2));
     const double nl = fma_(m0, t0.Pm, fma_(a1, t1.P0, l2 * t2.P1));
     vr = nr * rcp_nr(sr);
@@ -102,6 +109,8 @@ __device__ __forceinline__ Pair2 pair2(double a, double b)
     p.Lr = fma_(3.0, a, -b);
     return p;
 }
+// SCALED: vl and vr come out multiplied by 2.
+template <bool SCALED = false>
 __device__ __forceinline__ void recon2(double eps, const Pair2 &p1, const Pair2 &p0, double &vl,
                                        double &vr)
 {
@@ -110,7 +119,11 @@ __device__ __forceinline__ void recon2(double eps, const Pair2 &p1, const Pair2 
     q1 = q1 * q1;
     // right: d = (2/3, 1/3) ~ (2, 1): weights ~ 2 q1, q0; left: (1, 2): q1, 2 q0
     const double a0 = 2.0 * q1, b1 = 2.0 * q0;
-    const double sr = 2.0 * (a0 + q0), sl = 2.0 * (q1 + b1);
+    double sr = a0 + q0, sl = q1 + b1;
+    if constexpr (!SCALED) {
+        sr = 2.0 * sr;
+        sl = 2.0 * sl;
+    }
     const double nr = fma_(a0, p0.S, q0 * p1.Rr);
     const double nl = fma_(q1, p0.Lr, b1 * p1.S);
     vr = nr * rcp_nr(sr);
@@ -145,9 +158,15 @@ template <int R> __device__ __forceinline__ void store_run(double *p, const doub
 }
 
 // Stage combination in one form for every TVD stage: with L = rhs(us),
-//   un = post * (C*L + B*p1 + A*us + D*p2)      (post = 1/3 uses div3_fast)
-// evaluated as fma(D, p2, fma(C, L, fma(B, p1, A*us))).  p1/p2 are the pointwise operands
-// (u at the start of the step; or u_{n-4}, L(u_{n-4}) for the multistep method).
+//   un = post * (C*L + B*p1 + A*us + D*p2)
+// p1/p2 are the pointwise operands (u at the start of the step; or u_{n-4}, L(u_{n-4}) for
+// the multistep method).  The register-blocked kernels evaluate it with the factor post folded
+// into the coefficients.  In every stage with a pointwise operand the weights of us and p1 sum
+// to one (A' + B' = 1 with A' = A*post, B' = B*post), so it is taken as
+// fma(D', p2, fma(C', L, fma(A', us - p1, p1))): the rounding of a constant such as 2/3 then
+// scales only the O(dt) difference us - p1, and the state's total is not biased step after
+// step (which near a shock would move the shock).  The first stages (A = post = 1, no
+// pointwise operand) are fma(C, L, us).
 struct StageCoef {
     double A, B, C, D, post;
     bool div3;
@@ -233,11 +252,13 @@ __device__ __forceinline__ void interior_range(long long n, int R, int K, long l
 // 1/rx (the host keeps the line's edges within 1e8 cell widths of the origin, where every
 // width is rx to ~1e-8 and one step gives 1/w to a few ulp); otherwise by the hardware estimate
 // with two steps.
-template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR, bool NEWTON_W = true>
+template <int K, int FLUX, bool PERIODIC, int R, int NPW, bool INTERIOR, bool NEWTON_W = true,
+          bool CONSTW = false>
 __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, double alpha,
                                               const ChunkData<K, R, NPW> &d)
 {
     static_assert(R >= 4 && R % 2 == 0, "pairs of cells per lane");
+    static_assert(!CONSTW || NEWTON_W, "the constant width takes the Newton form's value");
     constexpr int H = K - 1;
     constexpr int NV = R + 2 * H;
     const int lane = threadIdx.x & 63;
@@ -245,7 +266,21 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     const int c0 = W + (lane - 1) * R;
     const bool owned = lane >= 1 && lane <= kOutLanes;
     const StageCoef sc = stage_coef(a.stage, a.h);
+    // post (1/3 for the stages flagged div3) folded into the coefficients (NPW >= 1; the
+    // NPW = 0 stages have A = post = 1)
+    const double post = sc.div3 ? 1.0 / 3.0 : sc.post;
+    [[maybe_unused]] const double cA = sc.A * post, cC = sc.C * post, cD = sc.D * post;
     const bool write_u = a.stage >= 0;
+    // the reconstructions come out scaled by S (recon2/recon3 <SCALED>); the flux constants
+    // take it back: f(v) = v^2/2 -> (v')^2 / (2 S^2), Lax-Friedrichs
+    // ((vm^2 + vp^2)/2 - alpha (vp - vm))/2 -> ((vm'^2 + vp'^2)/(4 S^2) - alpha/(2S) (vp' - vm'))
+    constexpr double S = K == 3 ? 6.0 : (K == 2 ? 2.0 : 1.0);
+    [[maybe_unused]] const double cq = 0.25 / (S * S), ca = (-0.5 / S) * alpha;
+    [[maybe_unused]] const double cg = 0.5 / (S * S);
+    auto numflux = [&](double vm, double vp) {
+        if constexpr (FLUX == GODUNOV) return godunov((vm * vm) * cg, (vp * vp) * cg, vm, vp);
+        else return fma_(cq, fma_(vm, vm, vp * vp), ca * (vp - vm));
+    };
 
     // ---- the lane's cells and window halo (v[i] <-> cell c0 - H + i) and pointwise operands
     double v[NV];
@@ -297,8 +332,13 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     };
     auto width_of = [&](int, double e_lo, double e_hi) { return e_hi - e_lo; };
     [[maybe_unused]] const double inv_rx = NEWTON_W ? rcp_nr(a.rx) : 0.0;
+    // CONSTW: every edge xmin + rx*i of the line is exact (checked on the host), so every
+    // width is rx and 1/width is the value the Newton form gives for w = rx, once per chunk
+    [[maybe_unused]] const double inv_w_const =
+        CONSTW ? fma_(inv_rx, fma_(-a.rx, inv_rx, 1.0), inv_rx) : 0.0;
     auto inv_width = [&](double w) {
-        if constexpr (NEWTON_W) return fma_(inv_rx, fma_(-w, inv_rx, 1.0), inv_rx);
+        if constexpr (CONSTW) return inv_w_const;
+        else if constexpr (NEWTON_W) return fma_(inv_rx, fma_(-w, inv_rx, 1.0), inv_rx);
         else return rcp_nr(w);
     };
 
@@ -314,11 +354,14 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         }
         L = (fleft - fright) * inv_width(w);
         const double us = v[H + t];
-        double tt = sc.A * us;
-        if constexpr (NPW >= 1) tt = fma_(sc.B, p1[t], tt);
-        tt = fma_(sc.C, L, tt);
-        if constexpr (NPW >= 2) tt = fma_(sc.D, p2[t], tt);
-        un = sc.div3 ? div3_fast(tt) : sc.post * tt;
+        if constexpr (NPW == 0) {
+            un = fma_(sc.C, L, us);
+        } else {
+            double tt = fma_(cA, us - p1[t], p1[t]);
+            tt = fma_(cC, L, tt);
+            if constexpr (NPW >= 2) tt = fma_(cD, p2[t], tt);
+            un = tt;
+        }
         if (real(g)) m_out = fmax(m_out, fabs(un));
     };
     auto store_pair = [&](int t, double un0, double un1, double L0, double L1) {   // cells t, t+1
@@ -357,7 +400,7 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         for (int t = 2; t < R; ++t) {
             const Tri3 t0 = tri3(v[t + 2], v[t + 3], v[t + 4]);   // T(c0+t)
             double vl, vr;
-            recon3(eps12, t2, t1, t0, vl, vr);
+            recon3<true>(eps12, t2, t1, t0, vl, vr);
             t2 = t1;
             t1 = t0;
             if (t == 2) {
@@ -365,7 +408,7 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
                 vr_prev = vr;
                 continue;
             }
-            const double f = burgers_numflux<FLUX, true>(vr_prev, vl, alpha);   // face c0+t-1/2
+            const double f = numflux(vr_prev, vl);   // face c0+t-1/2
             vr_prev = vr;
             if (t == 3) {
                 f_first = f;
@@ -402,13 +445,13 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         l1.P2 = __shfl_up(t1.P2, 1, 64);
         const double vr_left = __shfl_up(vr_prev, 1, 64);   // vr of cell c0-1 (lane-1's last)
         double vl0, vr0, vl1, vr1;
-        recon3(eps12, l2, l1, ta, vl0, vr0);
-        recon3(eps12, l1, ta, tb, vl1, vr1);
+        recon3<true>(eps12, l2, l1, ta, vl0, vr0);
+        recon3<true>(eps12, l1, ta, tb, vl1, vr1);
         const double vl_right = __shfl_down(vl0, 1, 64);    // vl of cell c0+R (lane+1's first)
-        const double f0 = burgers_numflux<FLUX, true>(vr_left, vl0, alpha);   // face c0-1/2
-        const double f1 = burgers_numflux<FLUX, true>(vr0, vl1, alpha);       // face c0+1/2
-        const double f2 = burgers_numflux<FLUX, true>(vr1, vl2, alpha);       // face c0+3/2
-        const double fR = burgers_numflux<FLUX, true>(vr_prev, vl_right, alpha);   // c0+R-1/2
+        const double f0 = numflux(vr_left, vl0);    // face c0-1/2
+        const double f1 = numflux(vr0, vl1);        // face c0+1/2
+        const double f2 = numflux(vr1, vl2);        // face c0+3/2
+        const double fR = numflux(vr_prev, vl_right);   // face c0+R-1/2
         const double e1 = edge(c0 + 1), e2 = edge(c0 + 2);
         double un0, L0, un1, L1, un2, L2;
         finish(0, f0, f1, width_of(c0, edge(c0), e1), un0, L0);
@@ -446,12 +489,12 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         double vl, vr;
         if constexpr (K == 3) {
             const Tri3 t0 = tri3(v[t + 2], v[t + 3], v[t + 4]);   // T(c0+t)
-            recon3(eps12, t2, t1, t0, vl, vr);
+            recon3<true>(eps12, t2, t1, t0, vl, vr);
             t2 = t1;
             t1 = t0;
         } else if constexpr (K == 2) {
             const Pair2 p0 = pair2(v[t + 1], v[t + 2]);
-            recon2(a.eps, pp1, p0, vl, vr);
+            recon2<true>(a.eps, pp1, p0, vl, vr);
             pp1 = p0;
         } else {
             vl = vr = v[t];
@@ -461,7 +504,7 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
             vr_prev = vr;
             continue;
         }
-        const double f = burgers_numflux<FLUX, true>(vr_prev, vl, alpha);   // face c0+t-1/2
+        const double f = numflux(vr_prev, vl);   // face c0+t-1/2
         vr_prev = vr;
         if (t == 1) {
             f_first = f;
@@ -494,13 +537,13 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
     const double vr_left = __shfl_up(vr_prev, 1, 64);    // vr of cell c0-1 (lane-1's last)
     const double vl_right = __shfl_down(vl0, 1, 64);     // vl of cell c0+R (lane+1's first)
     {
-        const double f0 = burgers_numflux<FLUX, true>(vr_left, vl0, alpha);
+        const double f0 = numflux(vr_left, vl0);
         double un, L;
         finish(0, f0, f_first, width_of(c0, e_c0, e_c1), un, L);
         store_pair(0, un, un1, L, L1);
     }
     {
-        const double fR = burgers_numflux<FLUX, true>(vr_prev, vl_right, alpha);
+        const double fR = numflux(vr_prev, vl_right);
         double un, L;
         finish(R - 1, fl, fR, width_of(c0 + R - 1, e_last, edge(c0 + R)), un, L);
         store_pair(R - 2, unR2, un, LR2, L);
@@ -565,7 +608,7 @@ __device__ __forceinline__ double alpha_reduce(const Stage1DArgs &a, unsigned lo
 // are unconditional (the last iteration reloads its own chunk, an L2 hit): a load on only some
 // paths would make the compiler's wait for the current chunk cover the prefetch as well.  The
 // few boundary chunks follow, one per wave.
-template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW, bool CONSTW>
 __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_red)
 {
     const unsigned long long part = alpha_partial<ALPHAMAX>(a);
@@ -584,7 +627,7 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
         for (; c <= hi; c += nw) {
             const long long c1 = c + nw <= hi ? c + nw : c;
             load_chunk<K, R, NPW>(a, (int)(c1 * cw), nxt);
-            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, (int)(c * cw), alpha, cur));
+            m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, true, true, CONSTW>(a, (int)(c * cw), alpha, cur));
             cur = nxt;
         }
     } else {
@@ -595,7 +638,7 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
     const long long nb = lo + (nch - 1 - hh);
     for (long long b = w; b < nb; b += nw) {
         const long long c = b < lo ? b : hh + 1 + (b - lo);
-        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, (int)(c * cw), alpha, cur));
+        m = fmax(m, chunk_stage<K, FLUX, PERIODIC, R, NPW, false, true, CONSTW>(a, (int)(c * cw), alpha, cur));
     }
     if constexpr (ALPHAMAX) publish_max(m, a.alpha_out, s_red);
 }
@@ -603,19 +646,19 @@ __device__ __forceinline__ void persistent_wave(const Stage1DArgs &a, double *s_
 // (The second register set of the prefetch costs one wave per SIMD against the plain form.
 // With the shared triples K = 3, R = 4 would fit four with at most one pointwise operand; it
 // measured no faster than three.)
-template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW, bool CONSTW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW) > 2 ? fast_waves(R, NPW) - 1 : 2)))
 void stage1d_pf_kernel(Stage1DArgs a)
 {
     __shared__ double s_red[kBlock / 64 + 1];
-    persistent_wave<K, FLUX, PERIODIC, ALPHAMAX, R, NPW>(a, s_red);
+    persistent_wave<K, FLUX, PERIODIC, ALPHAMAX, R, NPW, CONSTW>(a, s_red);
 }
 
 // One chunk per wave: wave w of block b owns the chunk starting at W = (4b + w)*62R.
 // When the line has interior chunks every wave loads one unconditionally (a boundary wave
 // the nearest interior chunk, unused), so that no join of paths with and without loads comes
 // before the alpha reduction, whose wait would then cover the chunk's loads.
-template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW, bool CONSTW>
 __device__ __forceinline__ void chunk_wave(const Stage1DArgs &a, double *s_red)
 {
     const long long cw = (long long)kOutLanes * R;
@@ -629,22 +672,22 @@ __device__ __forceinline__ void chunk_wave(const Stage1DArgs &a, double *s_red)
     if (lo <= hi) {
         load_chunk<K, R, NPW>(a, (int)((c < lo ? lo : (c > hi ? hi : c)) * cw), d);
         const double alpha = alpha_reduce<ALPHAMAX>(a, part);
-        if (c >= lo && c <= hi) m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, true>(a, W, alpha, d);
-        else m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, W, alpha, d);
+        if (c >= lo && c <= hi) m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, true, true, CONSTW>(a, W, alpha, d);
+        else m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, false, true, CONSTW>(a, W, alpha, d);
     } else {
         const double alpha = alpha_reduce<ALPHAMAX>(a, part);
-        m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, false>(a, W, alpha, d);
+        m_out = chunk_stage<K, FLUX, PERIODIC, R, NPW, false, true, CONSTW>(a, W, alpha, d);
     }
     if constexpr (ALPHAMAX) publish_max(m_out, a.alpha_out, s_red);
 }
 
 // NPW = stage_npw(a.stage), chosen by the host.
-template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW>
+template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW, bool CONSTW>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW))))
 void stage1d_fast_kernel(Stage1DArgs a)
 {
     __shared__ double s_red[kBlock / 64 + 1];
-    chunk_wave<K, FLUX, PERIODIC, ALPHAMAX, R, NPW>(a, s_red);
+    chunk_wave<K, FLUX, PERIODIC, ALPHAMAX, R, NPW, CONSTW>(a, s_red);
 }
 
 // --------------------------------------------------------------------------------------

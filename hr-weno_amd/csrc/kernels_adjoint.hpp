@@ -117,7 +117,8 @@ struct AdjChunk {
     long long off;              // the line's first cell in the flat state
     int line, start, n, k;      // line (-1: an idle padding chunk), first output cell, n, WENO k
     int first, nc;              // the line's chunks [first, first + nc)
-    double h, xmin, rx, pad;
+    double h, xmin, rx;
+    double constw;              // 1: every edge xmin + rx*i is exact, so every width is rx
 };
 static_assert(sizeof(AdjChunk) == 64, "one 64-byte record per chunk");
 
@@ -336,9 +337,9 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
                                                 const double *lam, const double *x1,
                                                 const double *x2, double *out, int n, int W,
                                                 double h, double xmin, double rx,
-                                                unsigned long long alpha_part, LineRec prec,
-                                                int line, long long off, int first, int nc,
-                                                unsigned *cand)
+                                                bool constw, unsigned long long alpha_part,
+                                                LineRec prec, int line, long long off,
+                                                int first, int nc, unsigned *cand)
 {
     constexpr int H = K - 1;
     constexpr int NV = R + 2 * H;
@@ -443,18 +444,20 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
             if (cell[i] == q0.j) mu[i] += d;
         if (edge && hc == q0.j) lam_edge += d;
     }
-    // phi = lam / width for the lane's cells, neighbours' from the adjacent lanes
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        const long long g = cell[i];
+    // phi = lam / width for the lane's cells, neighbours' from the adjacent lanes; on a grid
+    // whose edges are all exact every width is rx, and 1/rx is formed once (the same value)
+    const double iw_const = constw ? rcp_nr(rx) : 0.0;
+    auto inv_w = [&](long long g) {
+        if (constw) return iw_const;
         const double w = linear_edge(xmin, rx, g + 1) - linear_edge(xmin, rx, g);   // src/hrweno_grids.f90:76-78,247
-        phi[i + 1] = mu[i] * rcp_nr(w);
-    }
+        return rcp_nr(w);
+    };
+#pragma unroll
+    for (int i = 0; i < R; ++i) phi[i + 1] = mu[i] * inv_w(cell[i]);
     phi[0] = __shfl_up(phi[R], 1, 64);
     phi[R + 1] = __shfl_down(phi[1], 1, 64);
     if (edge) {
-        const double w = linear_edge(xmin, rx, hc + 1) - linear_edge(xmin, rx, hc);
-        const double ph = lam_edge * rcp_nr(w);
+        const double ph = lam_edge * inv_w(hc);
         if (lane == 0) phi[0] = ph;
         else phi[R + 1] = ph;
     }
@@ -582,7 +585,8 @@ void adjoint_stage_kernel(AdjArgs a)
     double ab;
 #define HRW_ADJ_CHUNK(KK, IN)                                                                    \
     chunk_adjoint<KK, R, ALPHAMAX, IN>(a, a.s + off, a.lam + off, x1, x2, a.out + off, n, W, h,  \
-                                        ch.xmin, ch.rx, apart, prec, line, off, first, nc,       \
+                                        ch.xmin, ch.rx, ch.constw != 0.0, apart, prec, line,     \
+                                        off, first, nc,                                          \
                                         &cand)
     if ((long long)W - R - (K - 1) >= 0 && (long long)W + 63LL * R + (K - 1) <= n) {
         if (K == 1) ab = HRW_ADJ_CHUNK(1, true);

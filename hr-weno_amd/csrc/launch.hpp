@@ -26,11 +26,15 @@ inline int device_cus()
 
 // Register-blocked 1D stage (FAST form).  form: one chunk per wavefront (stage1d_fast_kernel)
 // or persistent with the next chunk's loads issued ahead (stage1d_pf_kernel); r cells per lane
-// (4, 6, 8).  One definition per K (launch_fast_k<K>.o).
+// (4, 6, 8); constw: every edge of the linear grid is exact, so every width is rx.  One
+// definition per K (launch_fast_k<K>.o).
 enum { FORM_CHUNK = 0, FORM_PERSISTENT = 1 };
-void launch_fast_1d_k1(int flux, bool periodic, bool alphamax, int form, int r, const Stage1DArgs &a, hipStream_t s);
-void launch_fast_1d_k2(int flux, bool periodic, bool alphamax, int form, int r, const Stage1DArgs &a, hipStream_t s);
-void launch_fast_1d_k3(int flux, bool periodic, bool alphamax, int form, int r, const Stage1DArgs &a, hipStream_t s);
+void launch_fast_1d_k1(int flux, bool periodic, bool alphamax, bool constw, int form, int r,
+                       const Stage1DArgs &a, hipStream_t s);
+void launch_fast_1d_k2(int flux, bool periodic, bool alphamax, bool constw, int form, int r,
+                       const Stage1DArgs &a, hipStream_t s);
+void launch_fast_1d_k3(int flux, bool periodic, bool alphamax, bool constw, int form, int r,
+                       const Stage1DArgs &a, hipStream_t s);
 // LDS-tile 1D stage (EXACT form, and the FAST LDS variant).
 void launch_tile_1d(int k, int flux, bool periodic, bool fast, bool alphamax, const Stage1DArgs &a,
                     hipStream_t s);

@@ -48,6 +48,26 @@ module hrweno_capi
          integer(c_int64_t), value :: incv, incl, incr
       end function
 
+      integer(c_int) function hrw_reconstruct_lines_host(ncells, k, eps, arith, cnu, nlines, &
+                                                         v, incv, ldv, vl, vr, inco, ldo) bind(C)
+         import :: c_int, c_double, c_ptr, c_int64_t
+         integer(c_int), value :: ncells, k, arith
+         real(c_double), value :: eps
+         type(c_ptr), value :: cnu
+         integer(c_int64_t), value :: nlines, incv, ldv, inco, ldo
+         real(c_double), intent(in) :: v(*)
+         real(c_double), intent(inout) :: vl(*), vr(*)
+      end function
+
+      integer(c_int) function hrw_reconstruct_lines_dev(ncells, k, eps, arith, nlines, v, incv, &
+                                                        ldv, vl, vr, inco, ldo, stream) bind(C)
+         import :: c_int, c_double, c_ptr, c_int64_t
+         integer(c_int), value :: ncells, k, arith
+         real(c_double), value :: eps
+         integer(c_int64_t), value :: nlines, incv, ldv, inco, ldo
+         type(c_ptr), value :: v, vl, vr, stream
+      end function
+
       ! ---- fluxes on device arrays
       integer(c_int) function hrw_flux_lax_friedrichs(fn, a, b, n, vm, vp, x, t, alpha, h, &
                                                       stream) bind(C)
@@ -112,6 +132,11 @@ module hrweno_capi
       integer(c_int) function hrw_device_free(ptr) bind(C)
          import :: c_int, c_ptr
          type(c_ptr), value :: ptr
+      end function
+
+      integer(c_int) function hrw_device_mem_info(free_bytes, total_bytes) bind(C)
+         import :: c_int, c_size_t
+         integer(c_size_t), intent(out) :: free_bytes, total_bytes
       end function
 
       integer(c_int) function hrw_memcpy_htod(dst, src, bytes, stream) bind(C)
@@ -218,6 +243,32 @@ module hrweno_capi
          type(c_ptr), value :: ode
          integer(c_int64_t), intent(out) :: neq
          integer(c_int), intent(out) :: order, fevals, istate
+      end function
+
+      integer(c_int) function hrw_ode_set_check(ode, check_finite) bind(C)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ode
+         integer(c_int), value :: check_finite
+      end function
+
+      integer(c_int) function hrw_ode_state_size(ode, bytes) bind(C)
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ode
+         integer(c_int64_t), intent(out) :: bytes
+      end function
+
+      integer(c_int) function hrw_ode_save(ode, u, t, buf, bytes) bind(C)
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: ode, u, buf
+         real(c_double), value :: t
+         integer(c_int64_t), value :: bytes
+      end function
+
+      integer(c_int) function hrw_ode_load(ode, u, t, buf, bytes) bind(C)
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: ode, u, buf
+         real(c_double), intent(out) :: t
+         integer(c_int64_t), value :: bytes
       end function
 
       integer(c_int) function hrw_ode_tape(ode, ckpt_every) bind(C)

@@ -12,7 +12,8 @@ module hrweno_weno
    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_ptr, c_null_ptr, c_loc
    use hrweno_kinds, only: rk
    use hrweno_capi, only: HRW_ARITH_EXACT, HRW_ARITH_FAST, hrw_weno_calc_cnu, &
-                          hrw_reconstruct_host, hrw_reconstruct_dev, hrw_message
+                          hrw_reconstruct_host, hrw_reconstruct_dev, hrw_message, &
+                          hrw_reconstruct_lines_host, hrw_reconstruct_lines_dev
    implicit none
    private
 
@@ -51,6 +52,8 @@ module hrweno_weno
    contains
       procedure, pass(self) :: reconstruct => weno_reconstruct
       procedure, pass(self) :: reconstruct_dev => weno_reconstruct_dev
+      procedure, pass(self) :: reconstruct_lines => weno_reconstruct_lines
+      procedure, pass(self) :: reconstruct_lines_dev => weno_reconstruct_lines_dev
    end type weno
 
    interface weno
@@ -169,5 +172,52 @@ contains
       if (rc /= 0) error stop hrw_message()
 
    end subroutine weno_reconstruct_dev
+
+   subroutine weno_reconstruct_lines(self, v, vl, vr, nlines, incv, ldv, inco, ldo)
+   !! %reconstruct on nlines lines of one array in one device launch: element i of line l
+   !! (1-based) is v(1 + (l-1)*ldv + (i-1)*incv), and likewise vl, vr with inco, ldo.  The
+   !! x1 lines of example 2's state u((j-1)*nc1 + i) are nlines = nc2, incv = 1, ldv = nc1;
+   !! its x2 lines nlines = nc1, incv = nc1, ldv = 1 (example/example2_pbe_2d_fv.f90:95-112).
+   !! Elements of vl, vr between the lines keep their values.
+      class(weno), intent(in), target :: self
+      real(rk), intent(in) :: v(:)
+      real(rk), intent(inout) :: vl(:), vr(:)
+      integer, intent(in) :: nlines, incv, ldv, inco, ldo
+      type(c_ptr) :: pc
+      integer(c_int) :: rc
+      integer(c_int64_t) :: nv, no
+
+      if (nlines < 1 .or. incv < 1 .or. inco < 1 .or. ldv < 0 .or. ldo < 0) &
+         error stop "Invalid input 'nlines' or strides."
+      nv = int(nlines - 1, c_int64_t)*ldv + int(self%ncells - 1, c_int64_t)*incv + 1
+      no = int(nlines - 1, c_int64_t)*ldo + int(self%ncells - 1, c_int64_t)*inco + 1
+      if (size(v, kind=c_int64_t) < nv .or. size(vl, kind=c_int64_t) < no .or. &
+          size(vr, kind=c_int64_t) < no) error stop "Invalid input 'v', 'vl' or 'vr': too small."
+      pc = c_null_ptr
+      if (.not. self%uniform_grid) pc = c_loc(self%cnu)
+      rc = hrw_reconstruct_lines_host(self%ncells, self%k, self%eps, self%arith, pc, &
+                                      int(nlines, c_int64_t), v, int(incv, c_int64_t), &
+                                      int(ldv, c_int64_t), vl, vr, int(inco, c_int64_t), &
+                                      int(ldo, c_int64_t))
+      if (rc /= 0) error stop hrw_message()
+   end subroutine weno_reconstruct_lines
+
+   subroutine weno_reconstruct_lines_dev(self, v, vl, vr, nlines, incv, ldv, inco, ldo, stream)
+   !! The same on device arrays (uniform grids), enqueued on `stream` (default: null stream).
+      class(weno), intent(in) :: self
+      type(c_ptr), intent(in) :: v, vl, vr
+      integer, intent(in) :: nlines, incv, ldv, inco, ldo
+      type(c_ptr), intent(in), optional :: stream
+      type(c_ptr) :: s
+      integer(c_int) :: rc
+      if (.not. self%uniform_grid) error stop "reconstruct_lines_dev: non-uniform grids use reconstruct_lines."
+      s = c_null_ptr
+      if (present(stream)) s = stream
+      rc = hrw_reconstruct_lines_dev(self%ncells, self%k, self%eps, self%arith, &
+                                     int(nlines, c_int64_t), v, int(incv, c_int64_t), &
+                                     int(ldv, c_int64_t), vl, vr, int(inco, c_int64_t), &
+                                     int(ldo, c_int64_t), s)
+      if (rc /= 0) error stop hrw_message()
+   end subroutine weno_reconstruct_lines_dev
 
 end module hrweno_weno

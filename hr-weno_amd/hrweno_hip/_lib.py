@@ -29,6 +29,7 @@ SIGNATURES = {
     "hrw_device_count": [PI],
     "hrw_device_alloc": [C.c_size_t, C.POINTER(P)],
     "hrw_device_free": [P],
+    "hrw_device_mem_info": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "hrw_memcpy_htod": [P, P, C.c_size_t, P],
     "hrw_memcpy_dtoh": [P, P, C.c_size_t, P],
     "hrw_memcpy_dtod": [P, P, C.c_size_t, P],
@@ -48,6 +49,8 @@ SIGNATURES = {
     "hrw_weno_calc_cnu": [I, I, PD, PD],
     "hrw_reconstruct_host": [I, I, D, I, PD, PD, L, PD, L, PD, L],
     "hrw_reconstruct_dev": [I, I, D, I, P, L, P, L, P, L, P],
+    "hrw_reconstruct_lines_host": [I, I, D, I, PD, L, PD, L, L, PD, PD, L, L],
+    "hrw_reconstruct_lines_dev": [I, I, D, I, L, P, L, L, P, P, L, L, P],
     "hrw_flux_lax_friedrichs": [I, D, D, L, P, P, P, D, D, P, P],
     "hrw_flux_godunov": [I, D, D, L, P, P, P, D, P, P],
     "hrw_grid_linear": [D, D, I, PD],
@@ -70,10 +73,15 @@ SIGNATURES = {
     "hrw_ode_set_stream": [P, P],
     "hrw_ode_integrate": [P, P, PD, D, D, I],
     "hrw_ode_info": [P, PL, PI, PI, PI],
+    "hrw_ode_set_check": [P, I],
+    "hrw_ode_state_size": [P, PL],
+    "hrw_ode_save": [P, P, D, P, L],
+    "hrw_ode_load": [P, P, PD, P, L],
     "hrw_batch_burgers_create": [L, PI, PI, PD, PD, PD, D, I, D, I, C.POINTER(P)],
     "hrw_batch_destroy": [P],
     "hrw_batch_size": [P, PL, PL],
     "hrw_batch_set_stream": [P, P],
+    "hrw_batch_synchronize": [P],
     "hrw_batch_step": [P, P, L, PD, I],
     "hrw_batch_step_per_line": [P, P, L, PD],
     "hrw_batch_tune": [P, P, PI],

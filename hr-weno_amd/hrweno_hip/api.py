@@ -40,6 +40,26 @@ def _dptr(t) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 
 
+def _dvec(t, n: int, name: str, device=None, strided: bool = False) -> C.c_void_p:
+    """Device pointer of a float64 CUDA tensor of exactly n elements, checked before any
+    launch: contiguous (or, with strided=True, a 1D view of any positive stride), on `device`
+    when given.  A short or strided array would otherwise become out-of-bounds or wrong device
+    accesses."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError(f"{name}: expected a float64 CUDA tensor")
+    if t.numel() != n:
+        raise HrwError(1, f"{name}: expected {n} values, got {t.numel()}")
+    if strided:
+        if t.dim() != 1 or (n > 1 and t.stride(0) < 1):
+            raise HrwError(1, f"{name}: expected a 1D view with a positive stride")
+    elif not t.is_contiguous():
+        raise HrwError(1, f"{name}: expected a contiguous tensor")
+    if device is not None and t.device != device:
+        raise HrwError(1, f"{name}: on {t.device}, expected {device}")
+    return C.c_void_p(t.data_ptr())
+
+
 def _np(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -110,13 +130,29 @@ class Weno:
             vl = torch.empty(self.ncells, dtype=torch.float64, device=v.device)
         if vr is None:
             vr = torch.empty(self.ncells, dtype=torch.float64, device=v.device)
-        call("hrw_weno_reconstruct", self._h, _dptr(v), v.stride(0), _dptr(vl), vl.stride(0),
-             _dptr(vr), vr.stride(0), _stream())
+        pv = _dvec(v, self.ncells, "v", strided=True)
+        pl = _dvec(vl, self.ncells, "vl", v.device, strided=True)
+        pr = _dvec(vr, self.ncells, "vr", v.device, strided=True)
+        call("hrw_weno_reconstruct", self._h, pv, v.stride(0), pl, vl.stride(0), pr,
+             vr.stride(0), _stream())
         return vl, vr
 
     def reconstruct_lines(self, v, nlines: int, incv: int, ldv: int, vl, vr, inco: int, ldo: int):
-        """All nlines lines of a 2D array in one launch (element i of line l at l*ld + i*inc)."""
-        call("hrw_weno_reconstruct_lines", self._h, int(nlines), _dptr(v), int(incv), int(ldv),
+        """All nlines lines of a 2D array in one launch (element i of line l at l*ld + i*inc).
+        The arrays are contiguous buffers that must hold the last element addressed."""
+        n, L = self.ncells, int(nlines)
+        if min(int(incv), int(inco)) < 1 or min(int(ldv), int(ldo)) < 0 or L < 1:
+            raise HrwError(1, "reconstruct_lines: strides must be positive")
+        need_v = (L - 1) * int(ldv) + (n - 1) * int(incv) + 1
+        need_o = (L - 1) * int(ldo) + (n - 1) * int(inco) + 1
+        for name, t, need in (("v", v, need_v), ("vl", vl, need_o), ("vr", vr, need_o)):
+            _dptr(t)
+            if not t.is_contiguous() or t.numel() < need:
+                raise HrwError(1, f"reconstruct_lines: {name} must be a contiguous buffer of at "
+                                  f"least {need} values")
+            if t.device != v.device:
+                raise HrwError(1, f"reconstruct_lines: {name} on {t.device}, v on {v.device}")
+        call("hrw_weno_reconstruct_lines", self._h, L, _dptr(v), int(incv), int(ldv),
              _dptr(vl), _dptr(vr), int(inco), int(ldo), _stream())
         return vl, vr
 
@@ -142,9 +178,10 @@ def lax_friedrichs(vm, vp, x=None, t=0.0, alpha=1.0, fn=FN_BURGERS, coef=(0.0, 1
     torch = _torch()
     if out is None:
         out = torch.empty_like(vm)
-    call("hrw_flux_lax_friedrichs", fn, float(coef[0]), float(coef[1]), vm.numel(), _dptr(vm),
-         _dptr(vp), _dptr(x) if x is not None else None, float(t), float(alpha), _dptr(out),
-         _stream())
+    n = vm.numel()
+    call("hrw_flux_lax_friedrichs", fn, float(coef[0]), float(coef[1]), n, _dvec(vm, n, "vm"),
+         _dvec(vp, n, "vp", vm.device), _dvec(x, n, "x", vm.device) if x is not None else None,
+         float(t), float(alpha), _dvec(out, n, "out", vm.device), _stream())
     return out
 
 
@@ -153,8 +190,10 @@ def godunov(vm, vp, x=None, t=0.0, fn=FN_BURGERS, coef=(0.0, 1.0), out=None):
     torch = _torch()
     if out is None:
         out = torch.empty_like(vm)
-    call("hrw_flux_godunov", fn, float(coef[0]), float(coef[1]), vm.numel(), _dptr(vm),
-         _dptr(vp), _dptr(x) if x is not None else None, float(t), _dptr(out), _stream())
+    n = vm.numel()
+    call("hrw_flux_godunov", fn, float(coef[0]), float(coef[1]), n, _dvec(vm, n, "vm"),
+         _dvec(vp, n, "vp", vm.device), _dvec(x, n, "x", vm.device) if x is not None else None,
+         float(t), _dvec(out, n, "out", vm.device), _stream())
     return out
 
 
@@ -243,7 +282,9 @@ class Problem:
         torch = _torch()
         if udot is None:
             udot = torch.empty_like(u)
-        call("hrw_problem_rhs", self._h, float(t), _dptr(u), _dptr(udot), _stream())
+        n = self.neq
+        call("hrw_problem_rhs", self._h, float(t), _dvec(u, n, "u"),
+             _dvec(udot, n, "udot", u.device), _stream())
         return udot
 
 
@@ -317,8 +358,33 @@ class _Ode:
             raise HrwError(1, "integrate: u must be a contiguous vector of neq values")
         tt = C.c_double(t)
         call("hrw_ode_set_stream", self._h, _stream())
-        call("hrw_ode_integrate", self._h, _dptr(u), C.byref(tt), float(tout), float(dt),
-             int(itask))
+        call("hrw_ode_integrate", self._h, _dvec(u, self.neq, "u"), C.byref(tt), float(tout),
+             float(dt), int(itask))
+        return tt.value
+
+    def set_check(self, check_finite: bool = True):
+        """Scan u for NaN/Inf at the end of every integrate call (the default); a hit raises
+        HrwError and sets istate = -1."""
+        call("hrw_ode_set_check", self._h, int(bool(check_finite)))
+        return self
+
+    def save(self, u, t: float) -> bytes:
+        """The object's state (fevals, istate, the multistep history) with u and t, as bytes
+        for resume in another object or process (:meth:`load`)."""
+        nb = C.c_int64()
+        call("hrw_ode_state_size", self._h, C.byref(nb))
+        buf = C.create_string_buffer(nb.value)
+        call("hrw_ode_set_stream", self._h, _stream())
+        call("hrw_ode_save", self._h, _dvec(u, self.neq, "u"), float(t), buf, nb.value)
+        return buf.raw
+
+    def load(self, state: bytes, u) -> float:
+        """Restore a state written by :meth:`save` (same method, order and neq) into this
+        object and u; returns t."""
+        tt = C.c_double()
+        call("hrw_ode_set_stream", self._h, _stream())
+        call("hrw_ode_load", self._h, _dvec(u, self.neq, "u"), C.byref(tt),
+             C.c_char_p(bytes(state)), len(state))
         return tt.value
 
 
@@ -416,10 +482,17 @@ class BurgersBatch:
         self._tstream = stream
         call("hrw_batch_set_stream", self._h, C.c_void_p(stream.cuda_stream))
 
+    def _u(self, u, name="u"):
+        p = _dvec(u, self.total, name)
+        if u.data_ptr() % 16:
+            raise HrwError(1, f"{name}: the batch state must be 16-byte aligned")
+        return p
+
     def tune(self, u) -> int:
         r = C.c_int()
+        pu = self._u(u)
         cur = self._enter()
-        call("hrw_batch_tune", self._h, _dptr(u), C.byref(r))
+        call("hrw_batch_tune", self._h, pu, C.byref(r))
         self._leave(cur)
         return r.value
 
@@ -437,13 +510,15 @@ class BurgersBatch:
         return r.value, f.value
 
     def step(self, u, nsteps: int, graph: bool = True):
+        pu = self._u(u)
         cur = self._enter()
-        call("hrw_batch_step", self._h, _dptr(u), int(nsteps), _pd(self.t), int(bool(graph)))
+        call("hrw_batch_step", self._h, pu, int(nsteps), _pd(self.t), int(bool(graph)))
         self._leave(cur)
 
     def step_per_line(self, u, nsteps: int):
+        pu = self._u(u)
         cur = self._enter()
-        call("hrw_batch_step_per_line", self._h, _dptr(u), int(nsteps), _pd(self.t))
+        call("hrw_batch_step_per_line", self._h, pu, int(nsteps), _pd(self.t))
         self._leave(cur)
 
 
@@ -469,6 +544,7 @@ class Adjoint:
             call("hrw_adjoint_create", system._h, int(order), float(dt), int(nsteps),
                  int(ckpt_every), C.byref(h))
         self._h = h
+        self._n = None
         self.token = 0          # bumps on every recorded forward
 
     def __del__(self):
@@ -483,6 +559,11 @@ class Adjoint:
     def _sync_out(self):
         if self.batch:
             self._system._leave(self._cur)
+
+    def _neq(self) -> int:
+        if self._n is None:
+            self._n = self.info()["neq"]
+        return self._n
 
     def set_cells_per_lane(self, r: int):
         """Cells per lane of the reverse kernel (2 or 4); results agree to rounding."""
@@ -500,8 +581,10 @@ class Adjoint:
         """u_N from u0 (device tensors), recording the checkpoints."""
         if uN is None:
             uN = _torch().empty_like(u0)
+        n = self._neq()
+        p0, pN = _dvec(u0, n, "u0"), _dvec(uN, n, "uN", u0.device)
         self._sync_in()
-        call("hrw_adjoint_forward", self._h, _dptr(u0), _dptr(uN))
+        call("hrw_adjoint_forward", self._h, p0, pN)
         self._sync_out()
         self.token += 1
         return uN
@@ -510,8 +593,12 @@ class Adjoint:
         """dJ/du_0 from dJ/du_N for the last recorded forward (g0 may be gN)."""
         if g0 is None:
             g0 = _torch().empty_like(gN)
+        n = self._neq()
+        pN, p0 = _dvec(gN, n, "gN"), _dvec(g0, n, "g0", gN.device)
+        if g0.data_ptr() % 16:
+            raise HrwError(1, "g0: the cotangent must be 16-byte aligned")
         self._sync_in()
-        call("hrw_adjoint_backward", self._h, _dptr(gN), _dptr(g0))
+        call("hrw_adjoint_backward", self._h, pN, p0)
         self._sync_out()
         return g0
 

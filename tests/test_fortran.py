@@ -55,3 +55,11 @@ def test_fortran_dropin_example1():
 def test_fortran_adjoint():
     """rktvd%tape / %adjoint on a fused problem: FD agreement, conserved total, checkpoints."""
     run("test_adjoint")
+
+
+@pytest.mark.gpu
+def test_fortran_handles_device_integrand_state_lines():
+    """Ownership of the drop-in integrators (1000 created and dropped with device memory flat,
+    copies share, a temporary problem stays alive), rktvd_dev/mstvd_dev == fused, save/load
+    resumes bit for bit, reconstruct_lines == reconstruct line by line (example 2's loops)."""
+    run("test_handles", timeout=300)

@@ -132,6 +132,46 @@ def test_config3_subset(oracle, hh, dev):
         assert np.max(np.abs(out[l] - ou)) <= TOL * scale
 
 
+def test_config3_full_ensemble(oracle, hh, dev):
+    """The benchmarked ensemble at its full size and horizon: 4096 lines of 256..65536 cells,
+    k in {2, 3}, 200 RK3 steps.  The batched path (autotuned form and width, graph replay)
+    equals the one-launch-per-line path bit for bit on every line, and 16 lines spread over the
+    length range stay within the FAST tolerance of the oracle."""
+    import torch
+    from hrweno_hip import workloads as W
+    steps = 200
+    cases = W.config3(steps=steps)
+    lens = np.array([c.n for c in cases])
+    ks = np.array([c.k for c in cases])
+    dts = np.array([c.dt for c in cases])
+    b = hh.BurgersBatch(lens, ks, dts)
+    u = b.pack([c.u0 for c in cases], dev)
+    b.tune(u)
+    b.step(u, steps, graph=True)
+    bp = hh.BurgersBatch(lens, ks, dts)
+    up = bp.pack([c.u0 for c in cases], dev)
+    bp.step_per_line(up, steps)
+    torch.cuda.synchronize()
+    assert torch.equal(u, up)
+    assert np.array_equal(b.t, bp.t)
+    out = b.unpack(u)
+    order = np.argsort(lens)
+    oracle.set_threads(16)
+    try:
+        for l in order[np.linspace(0, lens.size - 1, 16).astype(int)]:
+            c = cases[l]
+            ou, ot, st, fe = oracle.burgers_rk(c.u0, c.width, k=c.k,
+                                               tout=W.steps_tout(0.0, c.dt, steps), dt=c.dt)
+            assert st == steps and ot == b.t[l]
+            scale = np.abs(ou).max()
+            d = np.abs(out[l] - ou)
+            assert d.max() <= TOL * scale, (l, c.n, d.max() / scale)
+            big = np.abs(ou) >= 1e-3 * scale
+            assert np.max(d[big] / np.abs(ou[big])) <= TOL, (l, c.n)
+    finally:
+        oracle.set_threads(1)
+
+
 @pytest.mark.parametrize("form", [0, 1])
 def test_few_long_lines_match_per_line(hh, dev, form):
     """A few long lines: whole blocks of wavefronts on one line, whose line maxima are reduced in

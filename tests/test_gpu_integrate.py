@@ -426,6 +426,38 @@ def test_config4_full_size_exact_steps(oracle, hh, dev):
     assert np.array_equal(_h(u), ou)
 
 
+def test_config4_full_size_exact_500_steps_and_fast(oracle, hh, dev):
+    """The benchmarked 2D workload at its full size and horizon: 2048 x 2048, the multistep
+    method for 500 steps (4 RK3 starting steps + 496), zero inflow / upwind outflow.  The EXACT
+    form (the one timed for this workload) is bit-identical to the oracle over all 500 steps.
+    The FAST form stays within 1e-12 of it (per cell of max|u|, and relative on every cell
+    above 1e-3 max|u|) over the first 20 steps; beyond that the reference's own method
+    amplifies rounding (a 1-ulp change of u0 is ~1e-11 by step 200 and ~1e-1 by step 500 in
+    the reference's own arithmetic: test_oracle_tvd.py::test_pbe_multistep_amplifies_rounding),
+    so no other rounding can stay within 1e-12 of the reference over the whole horizon."""
+    from hrweno_hip import workloads as W
+    c = W.config4()
+    e, _, _ = W.linear_centres(0.0, 1.0, c.n1)
+    res = {}
+    for steps in (20, c.steps):
+        tout = W.steps_tout(0.0, c.dt, steps)
+        for arith in ("exact", "fast"):
+            ode = hh.MsTvd(hh.PBE2D(c.n1, c.n2, bc=3, arith=arith))
+            u = _t(c.u0, dev)
+            t = ode.integrate(u, 0.0, tout, c.dt)
+            assert ode.fevals == 12 + steps - 4
+            res[(arith, steps)] = (_h(u), t)
+    assert res[("fast", 20)][1] == res[("exact", 20)][1]
+    assert_close(res[("fast", 20)][0], res[("exact", 20)][0])
+    oracle.set_threads(16)
+    try:
+        ou, ot, fe = oracle.pbe_ms(c.u0, e, e, bc=1, tout=W.steps_tout(0.0, c.dt, c.steps), dt=c.dt)
+    finally:
+        oracle.set_threads(1)
+    assert res[("exact", c.steps)][1] == ot and fe == 12 + c.steps - 4
+    assert np.array_equal(res[("exact", c.steps)][0], ou)
+
+
 # ------------------------------------------------------------------------------ callbacks
 def test_callback_rk_linear_ode_closed_form(hh, dev):
     """rktvd(fu, neq, order) with a device callback fu = a*u (test/test_tvdode.f90)."""

@@ -374,3 +374,27 @@ def test_pbe_characteristics_convergence(oracle):
     assert errs[-1] < 2e-6
     bad, _ = _pbe_convergence(oracle, plant=1)
     assert max(bad) < 1.5, bad
+
+
+def test_pbe_multistep_amplifies_rounding(oracle):
+    """The reference's 5-step multistep method on the config-4 problem (WENO5, upwind growth
+    rates g1 = 1 + 0.5*x1, g2 = 0.5, CFL 0.4) amplifies rounding: in the reference's own
+    arithmetic a 1-ulp change of every cell of u0 stays at rounding level for 100 steps, is
+    ~1e-11 by step 200 and past 1e-3 by step 500 (on 256^2; the same CFL on 2048^2 behaves
+    alike).  So over the 500 steps of that workload only an evaluation order identical to the
+    reference's can agree with it to 1e-12."""
+    from hrweno_hip import workloads as W
+    c = W.config4(n=256)
+    e, _, _ = W.linear_centres(0.0, 1.0, 256)
+    u1 = np.nextafter(c.u0, np.inf)
+    diff = {}
+    oracle.set_threads(8)
+    try:
+        for steps in (100, 200, 500):
+            tout = W.steps_tout(0.0, c.dt, steps)
+            a, _, _ = oracle.pbe_ms(c.u0, e, e, bc=1, tout=tout, dt=c.dt)
+            b, _, _ = oracle.pbe_ms(u1, e, e, bc=1, tout=tout, dt=c.dt)
+            diff[steps] = np.abs(a - b).max()
+    finally:
+        oracle.set_threads(1)
+    assert diff[100] < 1e-13 and diff[200] < 1e-10 and diff[500] > 1e-3, diff
