@@ -12,6 +12,7 @@
 #include <cstring>
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -1371,6 +1372,88 @@ extern "C" int hrw_ode_load(hrw_ode_t ode, double *u, double *t, const void *buf
     ode->istate = ode->method == METHOD_MS && !h.history && h.istate >= 2 ? 1 : h.istate;
     *t = h.t;
     return HRW_OK;
+}
+
+// ------------------------------------------------------------------ no-allocation steps -
+// hrw_rktvd_steps_noalloc: the stage buffers and the alpha slot sets of its steps live in a
+// file-scope device array instead of allocations.  Each device holds its own copy of the
+// array, at its own address, until the process ends; host code reaches it only through the
+// pointer hipGetSymbolAddress gives on that device (the array's name in host code is a
+// stand-in, not a device address).
+constexpr long long kNoAllocMaxCells = 1LL << 20;
+constexpr long long kNoAllocSlotWords = 3LL * kNSlot * kSlotStride;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxCells + kNoAllocSlotWords];
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch of the current device, looked up once per device with that device current.
+static int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(HRW_ERR_STATE, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(HRW_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return HRW_OK;
+}
+
+extern "C" int hrw_rktvd_steps_noalloc(hrw_problem_t p, int order, double *u, int64_t nsteps,
+                                       double dt, hrw_stream_t stream)
+{
+    if (!p) return fail(HRW_ERR_ARG, "null problem");
+    if (!u) return fail(HRW_ERR_ARG, "null state");
+    if (p->kind != PROB_BURGERS1D)
+        return fail(HRW_ERR_ARG, "hrw_rktvd_steps_noalloc takes a 1D Burgers problem");
+    if (!(order >= 1 && order <= 3))
+        return fail(HRW_ERR_ARG, "Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.");
+    if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
+    if (!(dt != 0.0) || !std::isfinite(dt)) return fail(HRW_ERR_ARG, "dt must be finite and nonzero");
+    if (p->neq > kNoAllocMaxCells)
+        return fail(HRW_ERR_ARG,
+                    "hrw_rktvd_steps_noalloc: %lld cells exceed its scratch of %lld cells per line",
+                    (long long)p->neq, kNoAllocMaxCells);
+    double *scr = nullptr;
+    CALL(noalloc_scratch(&scr));
+    if (nsteps == 0) return HRW_OK;
+    // an integrator on the stack whose buffers point into the scratch (detached before it goes)
+    hrw_ode_s o;
+    o.method = METHOD_RK;
+    o.order = order;
+    o.neq = p->neq;
+    o.p = p;
+    o.arith = p->arith;
+    o.stream = S(stream);
+    o.ui = scr;
+    o.ui2 = scr + kNoAllocMaxCells;
+    o.slots = reinterpret_cast<unsigned long long *>(scr + 2 * kNoAllocMaxCells);
+    int seq = 0;
+    int rc = seed_alpha(p, o.slots, u, o.stream);   // clears all three slot sets first
+    double *cur = u, *other = o.ui2;
+    for (int64_t st = 0; rc == HRW_OK && st < nsteps; ++st) {   // the steps of rk_integrate
+        if (order == 1) {
+            rc = rk_step(&o, seq, 0.0, dt, cur, other);
+            std::swap(cur, other);
+        } else {
+            rc = rk_step(&o, seq, 0.0, dt, cur, nullptr);
+        }
+    }
+    if (rc == HRW_OK && cur != u) {
+        const hipError_t e = hipMemcpyAsync(u, cur, (size_t)o.neq * sizeof(double),
+                                            hipMemcpyDeviceToDevice, o.stream);
+        if (e != hipSuccess) rc = fail(HRW_ERR_HIP, "copy of the last stage failed: %s", hipGetErrorString(e));
+    }
+    o.ui = o.ui2 = nullptr;
+    o.slots = nullptr;
+    o.p = nullptr;
+    return rc;
 }
 
 // ------------------------------------------------------------------ batched lines -----

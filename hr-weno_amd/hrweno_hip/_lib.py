@@ -77,6 +77,7 @@ SIGNATURES = {
     "hrw_ode_state_size": [P, PL],
     "hrw_ode_save": [P, P, D, P, L],
     "hrw_ode_load": [P, P, PD, P, L],
+    "hrw_rktvd_steps_noalloc": [P, I, P, L, D, P],
     "hrw_batch_burgers_create": [L, PI, PI, PD, PD, PD, D, I, D, I, C.POINTER(P)],
     "hrw_batch_destroy": [P],
     "hrw_batch_size": [P, PL, PL],

@@ -240,6 +240,18 @@ int hrw_ode_state_size(hrw_ode_t ode, int64_t *bytes);
 int hrw_ode_save(hrw_ode_t ode, const double *u, double t, void *buf, int64_t bytes);
 int hrw_ode_load(hrw_ode_t ode, double *u, double *t, const void *buf, int64_t bytes);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid hipMalloc): nsteps SSP-RK steps of order 1..3
+ * with step dt of the 1D Burgers problem p on the device state u -- the steps
+ * hrw_ode_integrate takes on hrw_rktvd_create(p, order), bit for bit -- with the stage buffers
+ * and the Lax-Friedrichs alpha slot sets in a static device array of the library (16 MiB +
+ * 6 KiB per device, held until the process ends) instead of allocations.  At most 2^20 cells
+ * (larger problems are refused).  Allocates nothing, does not synchronise (no NaN/Inf scan),
+ * enqueues on stream.  Every call on a device shares that device's array: calls that overlap
+ * on one device (other streams or threads) race. */
+int hrw_rktvd_steps_noalloc(hrw_problem_t p, int order, double *u, int64_t nsteps, double dt,
+                            hrw_stream_t stream);
+
 /* ---------------------------------------------------------------- batched lines ---- */
 /* Ensemble of nlines independent periodic 1D Burgers lines (the many-lines use of
  * rktvd(fu, neq, 3)%integrate, src/hrweno_tvdode.f90:160-171, with the example-1 RHS):
