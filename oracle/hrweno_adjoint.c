@@ -121,15 +121,49 @@ static void cell_vjp(int k, double eps, const double *c, const cell_state *s, do
     }
 }
 
-int hro_burgers_rhs_vjp(int n, int k, double eps, double alpha_fixed, const double *width,
-                        const double *v, const double *mu, double *out)
+/* Index of window position i + m: wrapped on a periodic line, clamped (the constant
+ * extrapolation inside src/hrweno_weno.f90:155-170) otherwise. */
+static inline int win(long long i, int n, int bc)
+{
+    if (bc == HRO_BC_PERIODIC) return wrap(i, n);
+    return (int)(i < 0 ? 0 : (i >= n ? n - 1 : i));
+}
+
+/* Partial derivatives of the interface flux F(vm, vp) for f(v) = v^2/2 (f'(v) = v):
+ *   Lax-Friedrichs (src/hrweno_fluxes.f90:42-43): (vm + alpha)/2, (vp - alpha)/2, and
+ *     -(vp - vm)/2 for alpha;
+ *   Godunov over the two end states (src/hrweno_fluxes.f90:66-74): the derivative of the
+ *     branch the flux takes.  At vm == vp (constant data reconstructs exactly) both branches
+ *     give f(v), and near that point F is f of the upwind state whatever the perturbation
+ *     (v > 0: min/max of an increasing f picks vm's side; v < 0: vp's), so F is smooth there
+ *     with gradient (v, 0), (0, v), or (0, 0) at v = 0. */
+static void flux_partials(int flux, double alpha, double vm, double vp, double *dm, double *dp,
+                          double *da)
+{
+    if (flux == HRO_FLUX_LF) {
+        *dm = 0.5 * (vm + alpha);
+        *dp = 0.5 * (vp - alpha);
+        if (g_plant == 2) *dp = 0.5 * (vp + alpha);   /* planted: sign of alpha */
+        *da = -0.5 * (vp - vm);
+        return;
+    }
+    const double fm = (vm * vm) / 2, fp = (vp * vp) / 2;
+    int takem = vm == vp ? vm > 0 : (vm <= vp ? fm < fp : fm > fp);
+    if (g_plant == 4) takem = !takem;                 /* planted: the other end state */
+    *dm = takem ? vm : 0.0;
+    *dp = takem ? 0.0 : vp;   /* (at vm == vp == 0 both are 0) */
+    *da = 0.0;
+}
+
+int hro_burgers_rhs_vjp_ex(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                           const double *width, const double *v, const double *mu, double *out)
 {
     if (n < 1 || k < 1 || k > 3) return -1;
     double d[3], c[12];
     hro_weno_tables(k, d, c);
     double alpha = alpha_fixed;
     int jmax = -1;
-    if (alpha < 0) {                                   /* alpha = maxval(abs(v)) */
+    if (flux == HRO_FLUX_LF && alpha < 0) {           /* alpha = maxval(abs(v)) */
         alpha = 0.0;
         jmax = 0;
         for (int i = 0; i < n; ++i)
@@ -140,31 +174,51 @@ int hro_burgers_rhs_vjp(int n, int k, double eps, double alpha_fixed, const doub
     }
     cell_state *cs = (cell_state *)malloc((size_t)n * sizeof(cell_state));
     double *phi = (double *)malloc((size_t)n * sizeof(double));
-    if (!cs || !phi) {
+    double *gr = (double *)calloc((size_t)n, sizeof(double));
+    double *gl = (double *)calloc((size_t)n, sizeof(double));
+    if (!cs || !phi || !gr || !gl) {
         free(cs);
         free(phi);
+        free(gr);
+        free(gl);
         return -2;
     }
     const int km = k - 1;
+    /* with zero-gradient ends the reference overwrites the boundary fluxes with their
+     * neighbours (example1:103-104), so L of the two end cells is identically zero */
+    const int periodic = bc == HRO_BC_PERIODIC;
     for (int i = 0; i < n; ++i) {
         double xw[5];
-        for (int m = -km; m <= km; ++m) xw[m + 2] = v[wrap((long long)i + m, n)];
+        for (int m = -km; m <= km; ++m) xw[m + 2] = v[win((long long)i + m, n, bc)];
         cell_forward(k, eps, d, c, xw + 2, &cs[i]);
         phi[i] = mu[i] / width[i];
     }
+    if (!periodic && g_plant != 5) {                  /* planted 5: end cells kept */
+        phi[0] = 0.0;
+        phi[n - 1] = 0.0;
+    }
     for (int i = 0; i < n; ++i) out[i] = 0.0;
+    /* interface i+1/2 between cell i (its vr) and cell ip (its vl): every i on a periodic
+     * line, i = 0..n-2 otherwise; the cotangent of F_{i+1/2} is phi[ip] - phi[i] */
     double abar = 0.0;
+    const int nif = periodic ? n : n - 1;
+    for (int i = 0; i < nif; ++i) {
+        const int ip = periodic ? wrap((long long)i + 1, n) : i + 1;
+        const double G = phi[ip] - phi[i];
+        double dm, dp, da;
+        flux_partials(flux, alpha, cs[i].vr, cs[ip].vl, &dm, &dp, &da);
+        gr[i] += G * dm;
+        gl[ip] += G * dp;
+        abar += G * da;
+    }
     for (int i = 0; i < n; ++i) {
-        const int ip = wrap((long long)i + 1, n), im = wrap((long long)i - 1, n);
-        const double Gp = phi[ip] - phi[i];            /* cotangent of F_{i+1/2} */
-        const double Gm = phi[i] - phi[im];            /* cotangent of F_{i-1/2} */
-        const double gr = Gp * (0.5 * (cs[i].vr + alpha));
-        double gl = Gm * (0.5 * (cs[i].vl - alpha));
-        if (g_plant == 2) gl = Gm * (0.5 * (cs[i].vl + alpha));   /* planted: sign of alpha */
-        abar += Gp * (-0.5 * (cs[ip].vl - cs[i].vr));
         double xb[5];
-        cell_vjp(k, eps, c, &cs[i], gr, gl, xb);
-        for (int m = -km; m <= km; ++m) out[wrap((long long)i + m, n)] += xb[m + 2];
+        cell_vjp(k, eps, c, &cs[i], gr[i], gl[i], xb);
+        for (int m = -km; m <= km; ++m) {
+            const int j = g_plant == 6 ? wrap((long long)i + m, n)   /* planted: ends wrapped */
+                                       : win((long long)i + m, n, bc);
+            out[j] += xb[m + 2];
+        }
     }
     if (jmax >= 0 && g_plant != 3) {                   /* planted 3: alpha held fixed */
         const double sg = (v[jmax] > 0) - (v[jmax] < 0);
@@ -172,7 +226,16 @@ int hro_burgers_rhs_vjp(int n, int k, double eps, double alpha_fixed, const doub
     }
     free(cs);
     free(phi);
+    free(gr);
+    free(gl);
     return 0;
+}
+
+int hro_burgers_rhs_vjp(int n, int k, double eps, double alpha_fixed, const double *width,
+                        const double *v, const double *mu, double *out)
+{
+    return hro_burgers_rhs_vjp_ex(n, k, eps, HRO_FLUX_LF, HRO_BC_PERIODIC, alpha_fixed, width,
+                                  v, mu, out);
 }
 
 int hro_burgers_rhs(int n, int k, double eps, int flux, int bc, double alpha_fixed,
@@ -189,10 +252,11 @@ int hro_burgers_rhs(int n, int k, double eps, int flux, int bc, double alpha_fix
     return 0;
 }
 
-/* The forward of hro_rktvd_integrate (periodic Burgers, LF) for nsteps steps, storing the
- * stage inputs, then the reverse sweep.  u <- u_N;  g: in dJ/du_N, out dJ/du_0. */
-int hro_burgers_rk_adjoint(int n, int k, double eps, double alpha_fixed, const double *width,
-                           int order, int64_t nsteps, double dt, double *u, double *g)
+/* The forward of hro_rktvd_integrate for nsteps steps, storing the stage inputs, then the
+ * reverse sweep.  u <- u_N;  g: in dJ/du_N, out dJ/du_0. */
+int hro_burgers_rk_adjoint_ex(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                              const double *width, int order, int64_t nsteps, double dt,
+                              double *u, double *g)
 {
     if (n < 1 || k < 1 || k > 3 || order < 1 || order > 3 || nsteps < 0) return -1;
     const size_t nn = (size_t)n;
@@ -202,7 +266,7 @@ int hro_burgers_rk_adjoint(int n, int k, double eps, double alpha_fixed, const d
     double *l2 = (double *)malloc(nn * sizeof(double));
     double *jt = (double *)malloc(nn * sizeof(double));
     hro_burgers_ctx ctx;
-    ctx.n = n; ctx.k = k; ctx.flux = HRO_FLUX_LF; ctx.bc = HRO_BC_PERIODIC; ctx.eps = eps;
+    ctx.n = n; ctx.k = k; ctx.flux = flux; ctx.bc = bc; ctx.eps = eps;
     ctx.alpha_fixed = alpha_fixed; ctx.width = width; ctx.cnu = NULL;
     ctx.work = (double *)malloc((size_t)5 * (nn + 2 * k + 2) * sizeof(double));
     int rc = 0;
@@ -232,29 +296,31 @@ int hro_burgers_rk_adjoint(int n, int k, double eps, double alpha_fixed, const d
             for (int i = 0; i < n; ++i) u[i] = ((u[i] + 2 * x2[i]) + (2 * dt) * udot[i]) / 3;
         }
     }
+#define VJP(x, m, o) hro_burgers_rhs_vjp_ex(n, k, eps, flux, bc, alpha_fixed, width, x, m, o)
     for (int64_t s = nsteps - 1; s >= 0; --s) {
         const double *x0 = X + (size_t)(order * s) * nn;
         if (order == 1) {                              /* u' = u + dt L(u) */
-            hro_burgers_rhs_vjp(n, k, eps, alpha_fixed, width, x0, g, jt);
+            VJP(x0, g, jt);
             for (int i = 0; i < n; ++i) g[i] = g[i] + dt * jt[i];
         } else if (order == 2) {                       /* u' = (u + u1 + dt L(u1)) / 2 */
-            hro_burgers_rhs_vjp(n, k, eps, alpha_fixed, width, x0 + nn, g, jt);
+            VJP(x0 + nn, g, jt);
             for (int i = 0; i < n; ++i) l1[i] = 0.5 * g[i] + 0.5 * dt * jt[i];
-            hro_burgers_rhs_vjp(n, k, eps, alpha_fixed, width, x0, l1, jt);
+            VJP(x0, l1, jt);
             for (int i = 0; i < n; ++i) g[i] = 0.5 * g[i] + l1[i] + dt * jt[i];
         } else {
             /* u' = (u + 2 u2 + 2 dt L(u2)) / 3 */
-            hro_burgers_rhs_vjp(n, k, eps, alpha_fixed, width, x0 + 2 * nn, g, jt);
+            VJP(x0 + 2 * nn, g, jt);
             for (int i = 0; i < n; ++i) l2[i] = (2.0 / 3) * g[i] + (2.0 / 3) * dt * jt[i];
             /* u2 = (3 u + u1 + dt L(u1)) / 4 */
-            hro_burgers_rhs_vjp(n, k, eps, alpha_fixed, width, x0 + nn, l2, jt);
+            VJP(x0 + nn, l2, jt);
             for (int i = 0; i < n; ++i) l1[i] = 0.25 * l2[i] + 0.25 * dt * jt[i];
             /* u1 = u + dt L(u) */
-            hro_burgers_rhs_vjp(n, k, eps, alpha_fixed, width, x0, l1, jt);
+            VJP(x0, l1, jt);
             for (int i = 0; i < n; ++i)
                 g[i] = (1.0 / 3) * g[i] + 0.75 * l2[i] + l1[i] + dt * jt[i];
         }
     }
+#undef VJP
 done:
     free(X);
     free(udot);
@@ -264,3 +330,128 @@ done:
     free(ctx.work);
     return rc;
 }
+
+int hro_burgers_rk_adjoint(int n, int k, double eps, double alpha_fixed, const double *width,
+                           int order, int64_t nsteps, double dt, double *u, double *g)
+{
+    return hro_burgers_rk_adjoint_ex(n, k, eps, HRO_FLUX_LF, HRO_BC_PERIODIC, alpha_fixed, width,
+                                     order, nsteps, dt, u, g);
+}
+
+/* The multistep integrator from a fresh object (istate 1, src/hrweno_tvdode.f90:203-271),
+ * S_0 = u -> S_N, storing every state, then the reverse sweep.
+ *   start (:236-249): S_{c+1} = RK3(S_c) for c = 0..3, each a single step of the RK3 method
+ *     of src/hrweno_tvdode.f90:160-171 (taken even past tout); L(S_c) is kept as udotold;
+ *   steps (:252-268): S_{m+1} = (25 S_m + 50 dt L(S_m) + 7 S_{m-4} + 10 dt L(S_{m-4}))/32 for
+ *     m = 4, 5, ... while not is_done(t, tout, dt).
+ * S_n feeds S_{n+1} (RK3 for n < 4, the multistep formula otherwise) and, as the oldest
+ * history column, S_{n+5}; so with lambda_m = dJ/dS_m (zero past N):
+ *   n >= 4: lambda_n = 25/32 l_{n+1} + 7/32 l_{n+5} + dt/32 J_n^T (50 l_{n+1} + 10 l_{n+5})
+ *   n <  4: lambda_n = RK3^T(S_n) l_{n+1} + 7/32 l_{n+5} + 10 dt/32 J_n^T l_{n+5}
+ * u <- S_N, *t advanced, *nsteps = N (0 if is_done(t0) already holds: no step, g kept). */
+int hro_burgers_ms_adjoint(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                           const double *width, double *t, double tout, double dt, double *u,
+                           double *g, int64_t *nsteps)
+{
+    if (n < 1 || k < 1 || k > 3 || !(dt != 0.0)) return -1;
+    *nsteps = 0;
+    if (hro_is_done(*t, tout, dt)) return 0;           /* :229 */
+    int64_t M = 0;                                     /* multistep steps, as the loop counts */
+    {
+        double tt = *t;
+        for (int c = 0; c < 4; ++c) tt = tt + dt;
+        while (!hro_is_done(tt, tout, dt)) {
+            tt = tt + dt;
+            ++M;
+        }
+    }
+    const int64_t N = 4 + M;
+    const size_t nn = (size_t)n;
+    double *S = (double *)malloc((size_t)(N + 1) * nn * sizeof(double));
+    double *Y = (double *)malloc((size_t)8 * nn * sizeof(double));   /* RK3 stages of the start */
+    double *lam = (double *)calloc((size_t)(N + 1) * nn, sizeof(double));
+    double *udot = (double *)malloc(nn * sizeof(double));
+    double *ud4 = (double *)malloc((size_t)(N + 1) * nn * sizeof(double));
+    double *mu = (double *)malloc(nn * sizeof(double));
+    double *jt = (double *)malloc(nn * sizeof(double));
+    double *l1 = (double *)malloc(nn * sizeof(double));
+    double *l2 = (double *)malloc(nn * sizeof(double));
+    hro_burgers_ctx ctx;
+    ctx.n = n; ctx.k = k; ctx.flux = flux; ctx.bc = bc; ctx.eps = eps;
+    ctx.alpha_fixed = alpha_fixed; ctx.width = width; ctx.cnu = NULL;
+    ctx.work = (double *)malloc((size_t)5 * (nn + 2 * k + 2) * sizeof(double));
+    int rc = 0;
+    if (!S || !Y || !lam || !udot || !ud4 || !mu || !jt || !l1 || !l2 || !ctx.work) {
+        rc = -2;
+        goto done;
+    }
+#define ST(m) (S + (size_t)(m) * nn)
+#define LM(m) (lam + (size_t)(m) * nn)
+    memcpy(ST(0), u, nn * sizeof(double));
+    for (int c = 0; c < 4; ++c) {                      /* :241-245 */
+        const double *x0 = ST(c);
+        double *x1 = Y + (size_t)(2 * c) * nn, *x2 = x1 + nn, *xo = ST(c + 1);
+        hro_rhs_burgers(&ctx, 0.0, x0, ud4 + (size_t)c * nn);   /* udotold(:, 4-c) */
+        hro_rhs_burgers(&ctx, 0.0, x0, udot);
+        for (int i = 0; i < n; ++i) x1[i] = x0[i] + dt * udot[i];
+        hro_rhs_burgers(&ctx, 0.0, x1, udot);
+        for (int i = 0; i < n; ++i) x2[i] = ((3 * x0[i] + x1[i]) + dt * udot[i]) / 4;
+        hro_rhs_burgers(&ctx, 0.0, x2, udot);
+        for (int i = 0; i < n; ++i) xo[i] = ((x0[i] + 2 * x2[i]) + (2 * dt) * udot[i]) / 3;
+        *t = *t + dt;
+    }
+    for (int64_t m = 4; m < N; ++m) {                  /* :255-266 */
+        const double *um = ST(m), *u4 = ST(m - 4), *d4 = ud4 + (size_t)(m - 4) * nn;
+        double *dm = ud4 + (size_t)m * nn;
+        hro_rhs_burgers(&ctx, 0.0, um, dm);
+        double *xo = ST(m + 1);
+        for (int i = 0; i < n; ++i)
+            xo[i] = ((((25 * um[i]) + (50 * dt) * dm[i]) + 7 * u4[i]) + (10 * dt) * d4[i]) / 32;
+        *t = *t + dt;
+    }
+    memcpy(u, ST(N), nn * sizeof(double));
+    memcpy(LM(N), g, nn * sizeof(double));
+    const double c7 = g_plant == 7 ? 0.0 : 7.0 / 32;   /* planted 7: history state dropped */
+    const double c10 = g_plant == 8 ? 0.0 : 10.0 / 32; /* planted 8: history slope dropped */
+#define VJP(x, m, o) hro_burgers_rhs_vjp_ex(n, k, eps, flux, bc, alpha_fixed, width, x, m, o)
+    for (int64_t m = N - 1; m >= 0; --m) {
+        const double *lp = LM(m + 1);
+        const double *l5 = m + 5 <= N ? LM(m + 5) : NULL;
+        double *lm = LM(m);
+        if (m >= 4) {
+            for (int i = 0; i < n; ++i) mu[i] = (50.0 / 32) * lp[i] + (l5 ? c10 * l5[i] : 0.0);
+            VJP(ST(m), mu, jt);
+            for (int i = 0; i < n; ++i)
+                lm[i] = (25.0 / 32) * lp[i] + (l5 ? c7 * l5[i] : 0.0) + dt * jt[i];
+        } else {
+            const double *x0 = ST(m), *x1 = Y + (size_t)(2 * m) * nn, *x2 = x1 + nn;
+            VJP(x2, lp, jt);
+            for (int i = 0; i < n; ++i) l2[i] = (2.0 / 3) * lp[i] + (2.0 / 3) * dt * jt[i];
+            VJP(x1, l2, jt);
+            for (int i = 0; i < n; ++i) l1[i] = 0.25 * l2[i] + 0.25 * dt * jt[i];
+            /* stage 1 of the RK3 step and the history term share J(S_m)^T */
+            for (int i = 0; i < n; ++i) mu[i] = l1[i] + (l5 ? c10 * l5[i] : 0.0);
+            VJP(x0, mu, jt);
+            for (int i = 0; i < n; ++i)
+                lm[i] = (1.0 / 3) * lp[i] + 0.75 * l2[i] + l1[i] + (l5 ? c7 * l5[i] : 0.0) +
+                        dt * jt[i];
+        }
+    }
+    memcpy(g, LM(0), nn * sizeof(double));
+    *nsteps = N;
+#undef ST
+#undef LM
+done:
+    free(S);
+    free(Y);
+    free(lam);
+    free(udot);
+    free(ud4);
+    free(mu);
+    free(jt);
+    free(l1);
+    free(l2);
+    free(ctx.work);
+    return rc;
+}
+#undef VJP

@@ -135,21 +135,36 @@ void hro_pbe_plant(int which);
 int hro_linear_ode(int n, const double *a, int order, double *u, double *t,
                    double tout, double dt);
 
-/* ---- discrete adjoint of the periodic Burgers path (hrweno_adjoint.c) ---- */
+/* ---- discrete adjoint of the Burgers path (hrweno_adjoint.c) ---- */
 /* One evaluation of hro_rhs_burgers (uniform tables). */
 int hro_burgers_rhs(int n, int k, double eps, int flux, int bc, double alpha_fixed,
                     const double *width, const double *v, double *vdot);
-/* out = (dL/dv)^T mu for L = hro_rhs_burgers with bc PERIODIC, the Lax-Friedrichs flux
- * and alpha = alpha_fixed (>= 0) or max|v| (alpha_fixed < 0). */
+/* out = (dL/dv)^T mu for L = hro_rhs_burgers with the given flux (Lax-Friedrichs with
+ * alpha = alpha_fixed >= 0 or max|v|, or Godunov) and ends (periodic or zero-gradient). */
+int hro_burgers_rhs_vjp_ex(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                           const double *width, const double *v, const double *mu, double *out);
+/* The periodic Lax-Friedrichs case of hro_burgers_rhs_vjp_ex. */
 int hro_burgers_rhs_vjp(int n, int k, double eps, double alpha_fixed, const double *width,
                         const double *v, const double *mu, double *out);
 /* nsteps fixed rktvd steps (order 1..3) of that RHS, u <- u_N, then the reverse sweep:
  * g in: dJ/du_N, out: dJ/du_0. */
+int hro_burgers_rk_adjoint_ex(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                              const double *width, int order, int64_t nsteps, double dt,
+                              double *u, double *g);
 int hro_burgers_rk_adjoint(int n, int k, double eps, double alpha_fixed, const double *width,
                            int order, int64_t nsteps, double dt, double *u, double *g);
-/* Test hook: 0 = exact adjoint; 1, 2, 3 plant one mistake each (weights held fixed, the sign
- * of alpha in the left-face flux derivative, alpha held fixed) so tests can show the checks
- * catch them. */
+/* mstvd%integrate from a fresh object (four RK3 starting steps, then the multistep method
+ * until is_done), u <- u_N, *t advanced, then the reverse sweep of g; *nsteps = N, the
+ * starting steps included. */
+int hro_burgers_ms_adjoint(int n, int k, double eps, int flux, int bc, double alpha_fixed,
+                           const double *width, double *t, double tout, double dt, double *u,
+                           double *g, int64_t *nsteps);
+/* Test hook: 0 = exact adjoint; each other value plants one mistake so tests can show the
+ * checks catch it: 1 weights held fixed, 2 the sign of alpha in the left-face flux
+ * derivative, 3 alpha held fixed, 4 Godunov differentiated on the other end state, 5 the
+ * zero-gradient end cells' RHS treated as live, 6 zero-gradient window cotangents wrapped
+ * around the line, 7 the multistep history state dropped, 8 the multistep history slope
+ * dropped. */
 void hro_adjoint_plant(int which);
 
 /* OpenMP thread count used by the per-cell loops (results do not depend on it). */

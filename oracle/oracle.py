@@ -75,6 +75,10 @@ class Oracle:
         lib.hro_burgers_rhs.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P]
         lib.hro_burgers_rhs_vjp.argtypes = [_I, _I, _D, _D, _P, _P, _P, _P]
         lib.hro_burgers_rk_adjoint.argtypes = [_I, _I, _D, _D, _P, _I, _L, _D, _P, _P]
+        lib.hro_burgers_rhs_vjp_ex.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _P, _P]
+        lib.hro_burgers_rk_adjoint_ex.argtypes = [_I, _I, _D, _I, _I, _D, _P, _I, _L, _D, _P, _P]
+        lib.hro_burgers_ms_adjoint.argtypes = [_I, _I, _D, _I, _I, _D, _P, _P, _D, _D, _P, _P,
+                                               C.POINTER(C.c_int64)]
         lib.hro_adjoint_plant.argtypes = [_I]
         lib.hro_set_threads.argtypes = [_I]
         lib.hro_get_threads.restype = _I
@@ -178,28 +182,44 @@ class Oracle:
             raise ValueError("hro_burgers_rhs failed")
         return out
 
-    # -- discrete adjoint (periodic Burgers, Lax-Friedrichs) ----------------------------
-    def burgers_rhs_vjp(self, v, mu, width, *, k=3, eps=1e-6, alpha=-1.0):
-        """(dL/dv)^T mu for the periodic Lax-Friedrichs Burgers RHS."""
+    # -- discrete adjoint (Burgers: Lax-Friedrichs or Godunov, periodic or zero-gradient) --
+    def burgers_rhs_vjp(self, v, mu, width, *, k=3, eps=1e-6, alpha=-1.0, flux=FLUX_LF,
+                        bc=BC_PERIODIC):
+        """(dL/dv)^T mu for the Burgers RHS."""
         v, mu, w = _f64(v), _f64(mu), _f64(width)
         out = np.empty(v.size)
-        if self.lib.hro_burgers_rhs_vjp(v.size, k, eps, alpha, _ptr(w), _ptr(v), _ptr(mu), _ptr(out)):
-            raise ValueError("hro_burgers_rhs_vjp failed")
+        if self.lib.hro_burgers_rhs_vjp_ex(v.size, k, eps, flux, bc, alpha, _ptr(w), _ptr(v),
+                                           _ptr(mu), _ptr(out)):
+            raise ValueError("hro_burgers_rhs_vjp_ex failed")
         return out
 
     def burgers_rk_adjoint(self, u0, gN, width, *, nsteps, dt, k=3, eps=1e-6, alpha=-1.0,
-                           order=3):
+                           order=3, flux=FLUX_LF, bc=BC_PERIODIC):
         """nsteps rktvd steps from u0 and the reverse sweep of gN; returns (u_N, dJ/du0)."""
         u = _f64(u0).copy()
         g = _f64(gN).copy()
         w = _f64(width)
-        if self.lib.hro_burgers_rk_adjoint(u.size, k, eps, alpha, _ptr(w), order, int(nsteps), dt,
-                                           _ptr(u), _ptr(g)):
-            raise ValueError("hro_burgers_rk_adjoint failed")
+        if self.lib.hro_burgers_rk_adjoint_ex(u.size, k, eps, flux, bc, alpha, _ptr(w), order,
+                                              int(nsteps), dt, _ptr(u), _ptr(g)):
+            raise ValueError("hro_burgers_rk_adjoint_ex failed")
         return u, g
 
+    def burgers_ms_adjoint(self, u0, gN, width, *, t0=0.0, tout, dt, k=3, eps=1e-6, alpha=-1.0,
+                           flux=FLUX_LF, bc=BC_PERIODIC):
+        """mstvd from a fresh object (t0 -> tout) and the reverse sweep of gN; returns
+        (u_N, dJ/du0, t, N) with N the steps taken, the four starting steps included."""
+        u = _f64(u0).copy()
+        g = _f64(gN).copy()
+        w = _f64(width)
+        t = C.c_double(t0)
+        ns = C.c_int64(0)
+        if self.lib.hro_burgers_ms_adjoint(u.size, k, eps, flux, bc, alpha, _ptr(w), C.byref(t),
+                                           tout, dt, _ptr(u), _ptr(g), C.byref(ns)):
+            raise ValueError("hro_burgers_ms_adjoint failed")
+        return u, g, t.value, int(ns.value)
+
     def adjoint_plant(self, which: int) -> None:
-        """Test hook: plant mistake `which` (1..3) in the adjoint, 0 restores it."""
+        """Test hook: plant mistake `which` (1..8) in the adjoint, 0 restores it."""
         self.lib.hro_adjoint_plant(int(which))
 
     def pbe_ms(self, u0, edges1, edges2, *, k=3, eps=1e-6, bc=1, g1=(1.0, 0.5), g2=(0.5, 0.0),
@@ -261,6 +281,7 @@ class Reference:
         lib.ref_weno_reconstruct_repeat.argtypes = [_I, _I, _D, _I, _P, _P, _P]
         lib.ref_fluxes.argtypes = [_I, _I, _I, _P, _P, _P, _D, _D, _P]
         lib.ref_burgers_rk3.argtypes = [_I, _I, _D, _I, _D, _P, _P]
+        lib.ref_burgers_ex.argtypes = [_I, _I, _D, _I, _I, _I, _I, _D, _P, _P]
         self.lib = lib
 
     def burgers_rk3(self, u0, width, nsteps, dt, k=3, eps=1e-6):
@@ -268,6 +289,16 @@ class Reference:
         u = _f64(u0).copy()
         w = _f64(width)
         self.lib.ref_burgers_rk3(u.size, k, eps, nsteps, dt, _ptr(w), _ptr(u))
+        return u
+
+    def burgers_ex(self, u0, width, nsteps, dt, *, k=3, eps=1e-6, flux=FLUX_LF,
+                   bc=BC_PERIODIC, method=3):
+        """Example 1's composition on the reference's own modules: nsteps RK3 steps
+        (method 3) or the multistep method with its four RK3 starting steps (method 0)."""
+        u = _f64(u0).copy()
+        w = _f64(width)
+        self.lib.ref_burgers_ex(u.size, k, eps, flux, bc, method, int(nsteps), dt, _ptr(w),
+                                _ptr(u))
         return u
 
     @staticmethod

@@ -136,4 +136,93 @@ contains
       end subroutine
    end subroutine
 
+   !> 1D Burgers with the composition of example/example1_burgers_1d_fv.f90:72-109 on the
+   !> reference's own weno%reconstruct and fluxes: bc 1 reconstructs the line itself (constant
+   !> extrapolation inside reconstruct), takes the flux at the n-1 interior interfaces and copies
+   !> the end ones (:97-104); bc 0 pads k periodic ghost cells per side.  flux 0:
+   !> lax_friedrichs with alpha = maxval(abs(v)), 1: godunov.  method 3: nsteps SSP-RK3 steps
+   !> (the stage lines of src/hrweno_tvdode.f90:162-167); method 0: the multistep method from
+   !> a fresh object, four RK3 starting steps then nsteps-4 steps of src/hrweno_tvdode.f90:
+   !> 236-266 (restated, as that module needs stdlib).  u advanced in place.
+   subroutine ref_burgers_ex(n, k, eps, flux, bc, method, nsteps, dt, width, u) &
+      bind(C, name="ref_burgers_ex")
+      integer(c_int), value :: n, k, flux, bc, method, nsteps
+      real(c_double), value :: eps, dt
+      real(c_double), intent(in) :: width(n)
+      real(c_double), intent(inout) :: u(n)
+      real(rk), allocatable :: ui(:), udot(:), vpad(:), vl(:), vr(:), fedges(:), uold(:, :), &
+                               udotold(:, :)
+      type(weno) :: w
+      real(rk) :: alpha
+      integer :: s, c
+      if (bc == 0) then
+         w = weno(n + 2*k, k, eps)
+         allocate (vpad(n + 2*k), vl(n + 2*k), vr(n + 2*k))
+      else
+         w = weno(n, k, eps)
+         allocate (vl(n), vr(n))
+      end if
+      allocate (ui(n), udot(n), fedges(0:n), uold(n, 4), udotold(n, 4))
+      if (method == 3) then
+         do s = 1, nsteps
+            call rk3_step()
+         end do
+      else
+         do c = 4, 1, -1
+            uold(:, c) = u
+            call rhs(u, udotold(:, c))
+            call rk3_step()
+         end do
+         do s = 5, nsteps
+            call rhs(u, udot)
+            ui = (25*u + 50*dt*udot + 7*uold(:, 4) + 10*dt*udotold(:, 4))/32
+            udotold = eoshift(udotold, shift=-1, dim=2)
+            uold = eoshift(uold, shift=-1, dim=2)
+            udotold(:, 1) = udot
+            uold(:, 1) = u
+            u = ui
+         end do
+      end if
+   contains
+      subroutine rk3_step()
+         call rhs(u, udot)
+         ui = u + dt*udot
+         call rhs(ui, udot)
+         ui = (3*u + ui + dt*udot)/4
+         call rhs(ui, udot)
+         u = (u + 2*ui + 2*dt*udot)/3
+      end subroutine
+      subroutine rhs(v, vdot)
+         real(rk), intent(in) :: v(:)
+         real(rk), intent(out) :: vdot(:)
+         integer :: i
+         alpha = maxval(abs(v))
+         if (bc == 0) then
+            vpad(1:k) = v(n - k + 1:n)
+            vpad(k + 1:k + n) = v
+            vpad(k + n + 1:) = v(1:k)
+            call w%reconstruct(vpad, vl, vr)
+            do i = 0, n
+               fedges(i) = iface(vr(k + i), vl(k + i + 1))
+            end do
+         else
+            call w%reconstruct(v, vl, vr)
+            do i = 1, n - 1
+               fedges(i) = iface(vr(i), vl(i + 1))
+            end do
+            fedges(0) = fedges(1)
+            fedges(n) = fedges(n - 1)
+         end if
+         vdot = -(fedges(1:) - fedges(:n - 1))/width
+      end subroutine
+      real(rk) function iface(vm, vp)
+         real(rk), intent(in) :: vm, vp
+         if (flux == 0) then
+            iface = lax_friedrichs(flux_burgers, vm, vp, [0.0_rk], 0.0_rk, alpha)
+         else
+            iface = godunov(flux_burgers, vm, vp, [0.0_rk], 0.0_rk)
+         end if
+      end function
+   end subroutine
+
 end module hrweno_ref_harness
