@@ -18,7 +18,12 @@ JSON line, each with its own `roofline` from this run's own timing:
                      the automatic tape (every stage input kept) and a checkpoint every 20 steps,
                      each as gradient time / plain forward time, and the reverse kernel's roofline;
   `adjoint_batch` -- the same over 256 lines of 1024..16384 cells vs one adjoint per line;
-  `pbe`           -- 2D population balance on 2048 x 2048, 5-step multistep method, 500 steps.
+  `adjoint_general` -- example 1's composition (Godunov, zero-gradient ends) at 2^20 cells x 400
+                     steps through the general reverse pass: the rktvd gradient and the recorded
+                     mstvd integrator's, each over its plain forward;
+  `pbe`           -- 2D population balance on 2048 x 2048, 5-step multistep method, 500 steps
+                     (the EXACT form: the multistep method amplifies rounding, so only the
+                     reference's own arithmetic meets 1e-12 over 500 steps).
 
 `--workload X` runs one leg alone as the headline (`--legs none` drops the sub-objects).
 
@@ -47,7 +52,8 @@ TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.json")
 
 
 # steps of each workload when it is not the headline (the configurations' own horizons)
-LEG_STEPS = {"plain": 100, "batch": 200, "pbe": 500, "adjoint": 400, "adjoint_batch": 400}
+LEG_STEPS = {"plain": 100, "batch": 200, "pbe": 500, "adjoint": 400, "adjoint_batch": 400,
+             "adjoint_general": 400}
 
 
 def release(torch) -> None:
@@ -285,7 +291,8 @@ def run_pbe(args, torch, hh, W, dev):
               "integrator": "mstvd (5-step, 3rd order)", "g1": "1+0.5*x1", "g2": "0.5",
               "bc": "zero inflow, upwind outflow", "arith": arith,
               "parity": "exact: bit-identical to the reference arithmetic over all 500 steps"
-                        if arith == "exact" else "fast: within 1e-12 for ~200 steps only",
+                        if arith == "exact" else "fast: within 1e-12 over ~20 steps only (the "
+                                                 "multistep method amplifies rounding)",
               "fast_cell_updates_per_s": ncell * args.steps / (out["fast"] / 1e3),
               "fast_roofline_frac": roofline(40.0 * ncell, out["fast"] / args.steps, "", "")["frac"],
               "parallelism": "single GPU"}
@@ -348,6 +355,65 @@ def run_adjoint(args, torch, hh, W, dev):
         "synthetic (seeded SplitMix64, config-1 style data; target from N(0,0.01) noise)"
 
 
+def run_adjoint_general(args, torch, hh, W, dev):
+    """Example 1's own composition (the endpoint Godunov flux, zero-gradient ends, WENO5,
+    example/example1_burgers_1d_fv.f90:72-109) at 2**20 cells x 400 steps through the general
+    reverse pass: the rktvd gradient of 0.5*|u_N - target|^2 (every stage input kept, so the
+    backward is the reverse launches alone) and the recorded mstvd integrator's (hrw_ode_tape /
+    hrw_ode_adjoint, the four RK3 starting steps included), each against its plain forward."""
+    c, noise = W.config5(steps=args.steps)
+    n, steps, dt = c.n, c.steps, c.dt
+    prob = hh.Burgers1D(n, flux=hh.GODUNOV, bc=hh.BC_ZERO_GRADIENT)
+    adj = hh.Adjoint(prob, nsteps=steps, dt=dt, ckpt_every=steps)
+    u0 = torch.as_tensor(c.u0, device=dev).clone()
+    target = adj.forward(torch.as_tensor(c.u0 + noise, device=dev)).clone()
+    res = {}
+    adj.backward(adj.forward(u0) - target)          # warm-up
+    ms_f = timed(torch, lambda: res.__setitem__("uN", adj.forward(u0)))
+    gN = res["uN"] - target
+    ms_b = timed(torch, lambda: res.__setitem__("g0", adj.backward(gN)))
+    if not bool(torch.isfinite(res["g0"]).all()):
+        raise RuntimeError("non-finite gradient")
+    tout = W.steps_tout(0.0, dt, steps)
+    ode = hh.RkTvd(prob, order=3)
+    up = u0.clone()
+    ms_p = timed(torch, lambda: ode.integrate(up, 0.0, tout, dt))
+    del adj
+
+    # mstvd: a recording starts with the method, so each recorded run is a fresh object
+    def ms_record():
+        o = hh.MsTvd(prob).tape(1)
+        u = u0.clone()
+        o.integrate(u, 0.0, tout, dt)
+        res["ms"] = (o, u)
+    ms_record()                                      # warm-up
+    ms_mf = timed(torch, ms_record)
+    o, um = res["ms"]
+    lam = um - target
+    ms_mb = timed(torch, lambda: o.adjoint(lam))
+    if not bool(torch.isfinite(lam).all()):
+        raise RuntimeError("non-finite gradient")
+    del res["ms"], o
+    om = hh.MsTvd(prob)
+    up = u0.clone()
+    ms_mp = timed(torch, lambda: om.integrate(up, 0.0, tout, dt))
+    log(f"adjoint (example 1 composition): rktvd forward(recording) {ms_f:.2f} ms, backward "
+        f"{ms_b:.2f} ms, plain forward {ms_p:.2f} ms; mstvd recording {ms_mf:.2f} ms, adjoint "
+        f"{ms_mb:.2f} ms, plain forward {ms_mp:.2f} ms")
+    rl = roofline(88.0 / 3 * n, ms_b / (3 * steps),
+                  "hrw::rhs_vjp_kernel<3,GODUNOV,zero-gradient,fixed>", "adjoint_general_2p20")
+    config = {"workload": "burgers1d_example1_godunov_zerograd_weno5_adjoint_2^20cells_400steps",
+              "cells": n, "steps": steps, "k": 3, "eps": 1e-6,
+              "loss": "0.5*|u_N - F(u0 + N(0,0.01))|^2", "parallelism": "single GPU",
+              "value_is": "cell-steps of the rktvd gradient (recording forward + backward) per s",
+              "gradient_over_forward": (ms_f + ms_b) / ms_p,
+              "backward_over_forward": ms_b / ms_p,
+              "mstvd_gradient_over_forward": (ms_mf + ms_mb) / ms_mp,
+              "mstvd_recording_ms": ms_mf, "mstvd_adjoint_ms": ms_mb, "mstvd_forward_ms": ms_mp}
+    return n * steps / ((ms_f + ms_b) / 1e3), (ms_f + ms_b) / steps, rl, config, \
+        "synthetic (seeded SplitMix64, config-1 style data; target from N(0,0.01) noise)"
+
+
 def run_adjoint_batch(args, torch, hh, W, dev):
     """The batched adjoint: 256 lines of 1024-16384 cells, every stage of every line in one
     launch, vs one adjoint object per line."""
@@ -397,7 +463,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["plain", "batch", "pbe", "adjoint", "adjoint_batch"],
+    ap.add_argument("--workload", choices=["plain", "batch", "pbe", "adjoint", "adjoint_batch",
+                                           "adjoint_general"],
                     default="plain")
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="adjoint checkpoint interval (0: the largest that fits in half of HBM)")
@@ -412,7 +479,7 @@ def main() -> int:
     ap.add_argument("--cells-per-lane", type=int, default=0,
                     help="cells per lane of the 1D stage kernel (0: chosen by the line length)")
     ap.add_argument("--per-line-steps", type=int, default=20)
-    ap.add_argument("--legs", default="batch,adjoint,adjoint_batch,pbe",
+    ap.add_argument("--legs", default="batch,adjoint,adjoint_batch,adjoint_general,pbe",
                     help="legs reported beside the headline ('none': the headline only)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of --cells doubles (20 times), the "
@@ -441,7 +508,7 @@ def main() -> int:
         return 0
 
     runners = {"plain": run_plain, "batch": run_batch, "pbe": run_pbe, "adjoint": run_adjoint,
-               "adjoint_batch": run_adjoint_batch}
+               "adjoint_batch": run_adjoint_batch, "adjoint_general": run_adjoint_general}
     value, ms_step, rl, config, data = runners[args.workload](args, torch, hh, W, dev)
     release(torch)
 

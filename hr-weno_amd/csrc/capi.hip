@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <climits>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -972,6 +974,77 @@ extern "C" int hrw_problem_rhs(hrw_problem_t p, double t, const double *u, doubl
 }
 
 // ------------------------------------------------------------------ integrators -------
+// ---- the general reverse pass of a 1D Burgers RHS (either flux, either kind of ends) ------
+// Scratch of rhs_vjp_kernel: a slot set for max|s|, one alpha-bar partial per block and the
+// argmax word (INT_MAX between passes: every fixup resets it).
+struct GenScratch {
+    unsigned long long *aset = nullptr;
+    double *partial = nullptr;
+    int *argmax = nullptr;
+    GenScratch() = default;
+    GenScratch(const GenScratch &) = delete;
+    GenScratch &operator=(const GenScratch &) = delete;
+    ~GenScratch()
+    {
+        dfree(aset);
+        dfree(partial);
+        dfree(argmax);
+    }
+    int alloc(int64_t n)
+    {
+        if (aset) return HRW_OK;
+        CALL(dalloc(&aset, (size_t)kNSlot * kSlotStride));
+        CALL(dalloc(&partial, (size_t)((n + kGenTile - 1) / kGenTile)));
+        CALL(dalloc(&argmax, 1));
+        const int none = INT_MAX;
+        HIP_TRY(hipMemcpy(argmax, &none, sizeof none, hipMemcpyHostToDevice));
+        return HRW_OK;
+    }
+};
+struct GenTerm {
+    const double *x;
+    double e;
+};
+// out = sum e_i x_i + ch J(s)^T (b1 y1 + b2 y2) for problem p (the Lax-Friedrichs alpha of s
+// taken as max|s| when the problem says so); out must not alias y1 or y2.
+static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const double *s,
+                   const double *y1, double b1, const double *y2, double b2,
+                   std::initializer_list<GenTerm> terms, double *out, double ch)
+{
+    GenAdjArgs a{};
+    a.s = s;
+    a.y1 = y1;
+    a.b1 = b1;
+    a.y2 = y2;
+    a.b2 = b2;
+    int i = 0;
+    for (const GenTerm &q : terms)
+        if (q.x && i < 4) {
+            a.x[i] = q.x;
+            a.e[i] = q.e;
+            ++i;
+        }
+    a.out = out;
+    a.ch = ch;
+    a.n = p->neq;
+    a.xmin = p->xmin;
+    a.rx = p->rx;
+    a.eps = p->eps;
+    a.alpha = p->alpha;
+    const bool amax = p->alphamax();
+    if (amax) {
+        clear_slots_kernel<<<1, 256, 0, st>>>(g.aset, 1);
+        maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, st>>>(s, p->neq, g.aset);
+        a.aset = g.aset;
+        a.partial = g.partial;
+        a.argmax = g.argmax;
+    }
+    launch_rhs_vjp(p->k, p->numflux, p->bc == HRW_BC_PERIODIC, amax, a, st);
+    if (amax) launch_rhs_vjp_fixup(a, st);
+    LAUNCH_CHECK();
+    return HRW_OK;
+}
+
 enum { METHOD_RK = 1, METHOD_MS = 2 };
 
 struct hrw_ode_s {
@@ -988,10 +1061,30 @@ struct hrw_ode_s {
     unsigned long long *slots = nullptr;
     int tape_every = 0;                    // > 0: integrate records for hrw_ode_adjoint
     hrw_adjoint_s *adj = nullptr;
+    // mstvd recording: every state since the method started (S_0 .. S_N), the two inner stage
+    // inputs of each of the four RK3 starting steps, each step's dt; the reverse sweep's
+    // cotangent ring (6) and stage cotangents (2), and the general pass's scratch
+    std::vector<double *> ms_tape;
+    std::vector<double *> ms_blocks;       // the tape's storage: one block per recorded call
+    double *ms_next = nullptr;             // next free state of the last block
+    int64_t ms_free = 0;                   // states left in it
+    double *ms_start[8] = {};
+    std::vector<double> ms_dts;
+    double *ms_lam[8] = {};
+    GenScratch gen;
     bool check_finite = true;              // integrate ends with a scan of u for NaN/Inf
     unsigned long long *d_bad = nullptr;   // the scan's result word
     ~hrw_ode_s();
 };
+static void ms_tape_clear(hrw_ode_s *o)
+{
+    for (auto &b : o->ms_blocks) dfree(b);
+    o->ms_blocks.clear();
+    o->ms_tape.clear();
+    o->ms_dts.clear();
+    o->ms_next = nullptr;
+    o->ms_free = 0;
+}
 static int ode_free_adjoint(hrw_ode_s *o);
 hrw_ode_s::~hrw_ode_s()
 {
@@ -1003,6 +1096,9 @@ hrw_ode_s::~hrw_ode_s()
     for (auto &b : ring_L) dfree(b);
     dfree(slots);
     dfree(d_bad);
+    ms_tape_clear(this);
+    for (auto &b : ms_start) dfree(b);
+    for (auto &b : ms_lam) dfree(b);
     ode_free_adjoint(this);
 }
 
@@ -1182,21 +1278,75 @@ static int rk_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
     return HRW_OK;
 }
 
+// Recording of mstvd%integrate for the adjoint: storage for the states a call adds (one
+// allocation per call, its step count known up front), and a copy of a new state on the tape.
+static int ms_reserve(hrw_ode_s *o, int64_t nstates)
+{
+    if (nstates <= 0) return HRW_OK;
+    double *b = nullptr;
+    CALL(dalloc(&b, (size_t)nstates * (size_t)o->neq));
+    o->ms_blocks.push_back(b);
+    o->ms_next = b;
+    o->ms_free = nstates;
+    return HRW_OK;
+}
+static int ms_record(hrw_ode_s *o, const double *src)
+{
+    if (o->ms_free <= 0) return fail(HRW_ERR_STATE, "multistep recording: tape storage exhausted");
+    double *b = o->ms_next;
+    o->ms_next += o->neq;
+    --o->ms_free;
+    o->ms_tape.push_back(b);
+    HIP_TRY(hipMemcpyAsync(b, src, o->neq * sizeof(double), hipMemcpyDeviceToDevice, o->stream));
+    return HRW_OK;
+}
+
 static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double dt)
 {
     if (o->istate < 1) return HRW_OK;                  // :228
     if (is_done(*t, tout, dt)) return HRW_OK;          // :229
+    // recording (hrw_ode_tape): the tape holds every state since the method started
+    const bool rec = o->tape_every > 0;
+    if (rec) {
+        if (o->istate == 1) ms_tape_clear(o);
+        else if (o->ms_tape.empty())
+            return fail(HRW_ERR_STATE, "recording a multistep integration starts with the method: "
+                                       "enable tape before the first integrate call (istate = 1)");
+        if (!o->ms_start[0])
+            for (auto &b : o->ms_start) CALL(dalloc(&b, (size_t)o->neq));
+        // the states this call adds: S_0 and the four starting steps' results when it starts
+        // the method, then one per step of the loop below (the same t accumulation)
+        int64_t add = 0;
+        double tt = *t;
+        if (o->istate == 1) {
+            add = 5;
+            for (int c = 0; c < 4; ++c) tt = tt + dt;
+        }
+        while (!is_done(tt, tout, dt)) {
+            tt = tt + dt;
+            ++add;
+        }
+        CALL(ms_reserve(o, add));
+    }
     const size_t bytes = o->neq * sizeof(double);
     HIP_TRY(hipMemcpyAsync(o->S, u, bytes, hipMemcpyDeviceToDevice, o->stream));
     int seq = 0;
     if (o->p) CALL(seed_alpha(o->p, o->slots, o->S, o->stream));
     auto col = [o](int c) { return (o->head + 4 - c) % 4; };   // column c (1 newest .. 4 oldest)
     if (o->istate == 1) {                              // :236-249, 4 RK3 starting steps
+        if (rec) CALL(ms_record(o, o->S));
         int fe = 0;
         for (int c = 4; c >= 1; --c) {
             HIP_TRY(hipMemcpyAsync(o->ring_u[col(c)], o->S, bytes, hipMemcpyDeviceToDevice, o->stream));
             CALL(ode_rhs(o, seq, *t, o->S, o->ring_L[col(c)]));
             CALL(rk_step(o, seq, *t, dt, o->S, nullptr));   // order 3
+            if (rec) {   // the step's inner stage inputs (still in ui, ui2) and its result
+                const int q = 2 * (4 - c);
+                HIP_TRY(hipMemcpyAsync(o->ms_start[q], o->ui, bytes, hipMemcpyDeviceToDevice, o->stream));
+                HIP_TRY(hipMemcpyAsync(o->ms_start[q + 1], o->ui2, bytes, hipMemcpyDeviceToDevice, o->stream));
+                CALL(ms_record(o, o->S));
+                o->ms_dts.push_back(dt);
+            }
             *t = *t + dt;
             fe += 3;
         }
@@ -1220,6 +1370,10 @@ static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
         o->fevals += 1;
         std::swap(o->S, o->ring_u[h4]);                // column 1 <- u_n, state <- u_{n+1}
         o->head = (h4 + 1) % 4;
+        if (rec) {
+            CALL(ms_record(o, o->S));
+            o->ms_dts.push_back(dt);
+        }
     }
     HIP_TRY(hipMemcpyAsync(u, o->S, bytes, hipMemcpyDeviceToDevice, o->stream));
     return HRW_OK;
@@ -1368,6 +1522,7 @@ extern "C" int hrw_ode_load(hrw_ode_t ode, double *u, double *t, const void *buf
     }
     HIP_TRY(hipStreamSynchronize(ode->stream));
     ode->fevals = h.fevals;
+    ms_tape_clear(ode);   // recorded states of another trajectory
     // a multistep state saved before its history existed starts over (istate 1)
     ode->istate = ode->method == METHOD_MS && !h.history && h.istate >= 2 ? 1 : h.istate;
     *t = h.t;
@@ -1911,6 +2066,8 @@ struct hrw_adjoint_s {
     std::vector<std::pair<const double *, Pending>> owed;   // buffer -> its pending correction
     int64_t y_seg = -1;           // segment whose stage inputs Y and the sets hold
     bool recorded = false;
+    bool general = false;         // one line that takes the general reverse pass
+    GenScratch gen;
     hipStream_t stream = nullptr;
     int nlines() const { return b ? b->nlines : 1; }
     hipStream_t strm() const { return b ? b->stream : stream; }
@@ -2076,11 +2233,18 @@ static int auto_ckpt(int64_t nsteps, int order, int64_t flat)
     return (int)lo;
 }
 
+// The adjoint covers the fused 1D Burgers problems on a linear grid, with either flux and either
+// kind of ends; periodic Lax-Friedrichs lines take the register-blocked reverse kernel
+// (adjoint_stage_kernel), the others the general pass (rhs_vjp_kernel).
 static bool adjoint_problem_ok(const hrw_problem_s *p)
 {
-    return p->kind == PROB_BURGERS1D && p->bc == HRW_BC_PERIODIC &&
-           p->numflux == HRW_FLUX_LAX_FRIEDRICHS && !p->d_width;
+    return p->kind == PROB_BURGERS1D && !p->d_width && !p->d_cnu;
 }
+static bool adjoint_fast_form(const hrw_problem_s *p)
+{
+    return p->bc == HRW_BC_PERIODIC && p->numflux == HRW_FLUX_LAX_FRIEDRICHS;
+}
+static const char *kAdjointCovers = "the adjoint covers 1D Burgers problems on a linear grid";
 
 extern "C" int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps,
                                   int ckpt_every, hrw_adjoint_t *out)
@@ -2088,8 +2252,7 @@ extern "C" int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t
     if (!out) return fail(HRW_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!p) return fail(HRW_ERR_ARG, "null problem");
-    if (!adjoint_problem_ok(p))
-        return fail(HRW_ERR_ARG, "the adjoint covers 1D Burgers, periodic, Lax-Friedrichs, linear grid");
+    if (!adjoint_problem_ok(p)) return fail(HRW_ERR_ARG, "%s", kAdjointCovers);
     if (!(order >= 1 && order <= 3))
         return fail(HRW_ERR_ARG, "Invalid input 'order' in 'rktvd'. Valid range: 1 <= k <= 3.");
     if (!(dt != 0.0) || !std::isfinite(dt)) return fail(HRW_ERR_ARG, "dt must be finite and nonzero");
@@ -2114,6 +2277,8 @@ extern "C" int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t
     CALL(upload(&a->d_xmin, std::vector<double>{p->xmin}));
     CALL(upload(&a->d_rx, std::vector<double>{p->rx}));
     CALL(adjoint_alloc(a, std::vector<int>{(int)p->neq}));
+    a->general = !adjoint_fast_form(p);
+    if (a->general) CALL(a->gen.alloc(p->neq));
     *out = guard.release();
     return HRW_OK;
 }
@@ -2125,7 +2290,7 @@ extern "C" int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_
     *out = nullptr;
     if (!b) return fail(HRW_ERR_ARG, "null batch");
     if (b->numflux != HRW_FLUX_LAX_FRIEDRICHS)
-        return fail(HRW_ERR_ARG, "the adjoint covers the Lax-Friedrichs flux");
+        return fail(HRW_ERR_ARG, "the batched adjoint covers the Lax-Friedrichs flux");
     if (nsteps < 0) return fail(HRW_ERR_ARG, "nsteps must be >= 0");
     if (ckpt_every < 0) return fail(HRW_ERR_ARG, "ckpt_every must be >= 0 (0 = automatic)");
     if (ckpt_every == 0) ckpt_every = auto_ckpt(nsteps, 3, b->total);
@@ -2258,6 +2423,9 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
                          double ca, double cc, const double *x1, double e1, const double *x2,
                          double e2)
 {
+    if (a->general)   // (its alpha-bar correction is applied within the pass: nothing owed)
+        return gen_vjp(a->p, a->gen, a->strm(), seg_state(a, s, q), lam, 1.0, nullptr, 0.0,
+                       {{lam, ca}, {x1, e1}, {x2, e2}}, out, cc * a->dt);
     AdjArgs g{};
     g.s = seg_state(a, s, q);
     g.lam = lam;
@@ -2410,14 +2578,58 @@ extern "C" int hrw_ode_tape(hrw_ode_t ode, int ckpt_every)
     if (!ode) return fail(HRW_ERR_ARG, "null ode");
     if (ckpt_every < 0) return fail(HRW_ERR_ARG, "ckpt_every must be >= 0");
     if (ckpt_every > 0) {
-        if (ode->method != METHOD_RK || !ode->p)
-            return fail(HRW_ERR_ARG, "recording needs rktvd on a device problem");
-        if (!adjoint_problem_ok(ode->p))
-            return fail(HRW_ERR_ARG, "the adjoint covers 1D Burgers, periodic, Lax-Friedrichs, linear grid");
+        if (!ode->p) return fail(HRW_ERR_ARG, "recording needs an integrator on a device problem");
+        if (!adjoint_problem_ok(ode->p)) return fail(HRW_ERR_ARG, "%s", kAdjointCovers);
     } else {
         ode_free_adjoint(ode);
+        ms_tape_clear(ode);
     }
     ode->tape_every = ckpt_every;
+    return HRW_OK;
+}
+
+// Reverse sweep of a recorded mstvd integration (src/hrweno_tvdode.f90:236-266): S_0 .. S_N on
+// the tape, step m (dt_m) maps S_m to S_{m+1} -- the RK3 method for m < 4, else
+// S_{m+1} = (25 S_m + 50 dt_m L(S_m) + 7 S_{m-4} + 10 dt_m L(S_{m-4}))/32 -- and S_m is the
+// oldest history column of step m + 4.  With l_m = dJ/dS_m (zero past N):
+//   m >= 4: l_m = 25/32 l_{m+1} + 7/32 l_{m+5} + J_m^T (50 dt_m/32 l_{m+1} + 10 dt_{m+4}/32 l_{m+5})
+//   m <  4: l_m = RK3^T(S_m) l_{m+1} + 7/32 l_{m+5} + 10 dt_{m+4}/32 J_m^T l_{m+5}, the last two
+//           terms joining the first stage's reverse (the same Jacobian)
+// one general pass per multistep step, three per starting step; l_m in ring slot m % 6.
+static int ms_adjoint(hrw_ode_s *o, double *lambda)
+{
+    const int64_t N = (int64_t)o->ms_tape.size() - 1;
+    if (N <= 0) return HRW_OK;
+    const hrw_problem_s *p = o->p;
+    hipStream_t st = o->stream;
+    const size_t bytes = (size_t)o->neq * sizeof(double);
+    if (!o->ms_lam[0])
+        for (auto &b : o->ms_lam) CALL(dalloc(&b, (size_t)o->neq));
+    CALL(o->gen.alloc(o->neq));
+    auto L = [o](int64_t m) { return o->ms_lam[m % 6]; };
+    double *L2 = o->ms_lam[6], *L1 = o->ms_lam[7];
+    const std::vector<double> &dts = o->ms_dts;
+    HIP_TRY(hipMemcpyAsync(L(N), lambda, bytes, hipMemcpyDeviceToDevice, st));
+    for (int64_t m = N - 1; m >= 0; --m) {
+        const double *lp = L(m + 1);
+        const double *l5 = m + 5 <= N ? L(m + 5) : nullptr;
+        const double d5 = l5 ? dts[(size_t)m + 4] : 0.0;
+        const double dm = dts[(size_t)m];
+        const double *Sm = o->ms_tape[(size_t)m];
+        if (m >= 4) {
+            CALL(gen_vjp(p, o->gen, st, Sm, lp, 50.0 * dm / 32.0, l5, 10.0 * d5 / 32.0,
+                         {{lp, 25.0 / 32.0}, {l5, 7.0 / 32.0}}, L(m), 1.0));
+        } else {
+            const double *Y1 = o->ms_start[2 * m], *Y2 = o->ms_start[2 * m + 1];
+            // u' = (u + 2 Y2 + 2 dt L(Y2))/3, Y2 = (3u + Y1 + dt L(Y1))/4, Y1 = u + dt L(u)
+            CALL(gen_vjp(p, o->gen, st, Y2, lp, 1.0, nullptr, 0.0, {{lp, 2.0 / 3.0}}, L2,
+                         2.0 / 3.0 * dm));
+            CALL(gen_vjp(p, o->gen, st, Y1, L2, 1.0, nullptr, 0.0, {{L2, 0.25}}, L1, 0.25 * dm));
+            CALL(gen_vjp(p, o->gen, st, Sm, L1, dm, l5, 10.0 * d5 / 32.0,
+                         {{lp, 1.0 / 3.0}, {L2, 0.75}, {L1, 1.0}, {l5, 7.0 / 32.0}}, L(m), 1.0));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(lambda, L(0), bytes, hipMemcpyDeviceToDevice, st));
     return HRW_OK;
 }
 
@@ -2425,6 +2637,11 @@ extern "C" int hrw_ode_adjoint(hrw_ode_t ode, double *lambda)
 {
     if (!ode) return fail(HRW_ERR_ARG, "null ode");
     if (!lambda) return fail(HRW_ERR_ARG, "null lambda");
+    if (ode->method == METHOD_MS) {
+        if (ode->ms_tape.empty())
+            return fail(HRW_ERR_STATE, "adjoint: no recorded integrate call (enable it with tape)");
+        return ms_adjoint(ode, lambda);
+    }
     if (!ode->adj || !ode->adj->recorded)
         return fail(HRW_ERR_STATE, "adjoint: no recorded integrate call (enable it with tape)");
     ode->adj->stream = ode->stream;

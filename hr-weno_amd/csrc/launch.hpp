@@ -8,6 +8,7 @@
 #include "kernels_1d_fast.hpp"
 #include "kernels_2d.hpp"
 #include "kernels_adjoint.hpp"
+#include "kernels_adjoint_general.hpp"
 
 namespace hrw {
 
@@ -42,7 +43,12 @@ void launch_tile_1d(int k, int flux, bool periodic, bool fast, bool alphamax, co
 void launch_pbe2d(int k, int bc, bool fast, const Stage2DArgs &a, hipStream_t s);
 // One stage of every line of a batch (form HRW_BATCH_CHUNK or HRW_BATCH_PERSISTENT).
 void launch_batch_stage(int flux, bool alphamax, int R, int form, const BatchArgs &a, hipStream_t s);
-// The reverse of one stage.
+// The reverse of one stage (periodic Lax-Friedrichs lines, register-blocked).
 void launch_adjoint_stage(int R, bool alphamax, const AdjArgs &g, hipStream_t s);
+// out = sum e_i x_i + ch J(s)^T (b1 y1 + b2 y2) for any 1D Burgers composition on a linear grid,
+// and (alpha = max|s|) the alpha-bar correction once that pass has finished.
+void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjArgs &g,
+                    hipStream_t s);
+void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s);
 
 }  // namespace hrw

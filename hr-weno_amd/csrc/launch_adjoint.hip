@@ -1,4 +1,4 @@
-// launch_adjoint.hip -- launcher of the reverse stage kernel.
+// launch_adjoint.hip -- launchers of the reverse stage kernels.
 #include "launch.hpp"
 
 namespace hrw {
@@ -11,7 +11,44 @@ static void ladj(const AdjArgs &g, hipStream_t s)
     adjoint_stage_kernel<R, AMAX><<<grid, kBlock, 0, s>>>(g);
 }
 
+template <int K, int FLUX, bool PER, bool AMAX>
+static void lgen(const GenAdjArgs &g, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((g.n + kGenTile - 1) / kGenTile);
+    rhs_vjp_kernel<K, FLUX, PER, AMAX><<<grid, kGenTile, 0, s>>>(g);
+}
+template <int K>
+static void lgen_k(int flux, bool periodic, bool amax, const GenAdjArgs &g, hipStream_t s)
+{
+    if (flux == LF) {
+        if (periodic) {
+            if (amax) lgen<K, LF, true, true>(g, s);
+            else lgen<K, LF, true, false>(g, s);
+        } else {
+            if (amax) lgen<K, LF, false, true>(g, s);
+            else lgen<K, LF, false, false>(g, s);
+        }
+    } else {
+        if (periodic) lgen<K, GODUNOV, true, false>(g, s);
+        else lgen<K, GODUNOV, false, false>(g, s);
+    }
+}
+
 }  // namespace
+
+void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjArgs &g,
+                    hipStream_t s)
+{
+    if (k == 1) lgen_k<1>(flux, periodic, alphamax, g, s);
+    else if (k == 2) lgen_k<2>(flux, periodic, alphamax, g, s);
+    else lgen_k<3>(flux, periodic, alphamax, g, s);
+}
+
+void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s)
+{
+    const int nblocks = (int)((g.n + kGenTile - 1) / kGenTile);
+    rhs_vjp_alpha_fixup<<<1, kGenTile, 0, s>>>(g.partial, nblocks, g.argmax, g.s, g.out, g.ch);
+}
 
 void launch_adjoint_stage(int R, bool alphamax, const AdjArgs &g, hipStream_t s)
 {

@@ -94,15 +94,15 @@ module hrweno_tvdode
       procedure, pass(self) :: load_state => tvdode_load_state
       procedure, pass(self) :: save_state_dev => tvdode_save_state_dev
       procedure, pass(self) :: load_state_dev => tvdode_load_state_dev
+      procedure, pass(self) :: tape => tvdode_tape
+      procedure, pass(self) :: adjoint => tvdode_adjoint
+      procedure, pass(self) :: adjoint_dev => tvdode_adjoint_dev
    end type tvdode
 
    type, extends(tvdode) :: rktvd
    !! Runge-Kutta TVD solver.
    contains
       procedure, pass(self) :: integrate => rktvd_integrate
-      procedure, pass(self) :: tape => rktvd_tape
-      procedure, pass(self) :: adjoint => rktvd_adjoint
-      procedure, pass(self) :: adjoint_dev => rktvd_adjoint_dev
    end type rktvd
 
    type, extends(tvdode) :: mstvd
@@ -390,26 +390,29 @@ contains
       call integrate_host(self, u, t, tout, dt, 1)
    end subroutine mstvd_integrate
 
-   subroutine rktvd_tape(self, ckpt_every)
-   !! Record the following %integrate calls for %adjoint, keeping a device checkpoint every
-   !! ckpt_every steps (0 stops recording).  Needs rktvd(prob, order) on a periodic
-   !! Lax-Friedrichs burgers1d problem.  (Added form: the reference has no adjoint.)
-      class(rktvd), intent(inout) :: self
+   subroutine tvdode_tape(self, ckpt_every)
+   !! Record the following %integrate calls for %adjoint (0 stops recording).  rktvd records
+   !! its last %integrate call with a device checkpoint every ckpt_every steps; mstvd keeps
+   !! every state from the call that starts the method (istate = 1) on, so it must be set
+   !! before that call.  Needs an integrator on a burgers1d problem with a linear grid (either
+   !! flux, either kind of ends).  (Added form: the reference has no adjoint.)
+      class(tvdode), intent(inout) :: self
       integer, intent(in) :: ckpt_every
       if (hrw_ode_tape(self%r%h, int(ckpt_every, c_int)) /= 0) call self%error_msg(hrw_message())
-   end subroutine rktvd_tape
+   end subroutine tvdode_tape
 
-   subroutine rktvd_adjoint_dev(self, lambda)
+   subroutine tvdode_adjoint_dev(self, lambda)
    !! %adjoint on a device array lambda (neq doubles).
-      class(rktvd), intent(inout) :: self
+      class(tvdode), intent(inout) :: self
       type(c_ptr), intent(in) :: lambda
       if (hrw_ode_adjoint(self%r%h, lambda) /= 0) call self%error_msg(hrw_message())
-   end subroutine rktvd_adjoint_dev
+   end subroutine tvdode_adjoint_dev
 
-   subroutine rktvd_adjoint(self, lambda)
-   !! Discrete adjoint of the last recorded %integrate call: lambda holds dJ/du at its final
-   !! state on entry and dJ/du at its initial state on return.
-      class(rktvd), intent(inout) :: self
+   subroutine tvdode_adjoint(self, lambda)
+   !! Discrete adjoint of the recording: lambda holds dJ/du at its final state on entry and
+   !! dJ/du at its initial state on return (rktvd: the start of the last recorded %integrate
+   !! call; mstvd: the start of the method).
+      class(tvdode), intent(inout) :: self
       real(rk), intent(inout) :: lambda(:)
       real(rk), allocatable, target :: w(:)
       integer(c_size_t) :: bytes
@@ -422,7 +425,7 @@ contains
       if (hrw_memcpy_dtoh(c_loc(w), self%r%ubuf, bytes, c_null_ptr) /= 0) &
          call self%error_msg(hrw_message())
       lambda = w
-   end subroutine rktvd_adjoint
+   end subroutine tvdode_adjoint
 
    subroutine tvdode_check_finite(self, on)
    !! Switch the NaN/Inf scan that ends every %integrate on (the default) or off.

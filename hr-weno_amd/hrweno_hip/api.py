@@ -362,6 +362,21 @@ class _Ode:
              float(dt), int(itask))
         return tt.value
 
+    def tape(self, ckpt_every: int = 1):
+        """Record the following integrate calls for :meth:`adjoint` (0 stops).  rktvd records
+        its last integrate call with a checkpoint every ``ckpt_every`` steps; mstvd keeps every
+        state from the call that starts the method (istate 1) on, so tape must be set before
+        that call.  Needs a device problem (1D Burgers on a linear grid)."""
+        call("hrw_ode_tape", self._h, int(ckpt_every))
+        return self
+
+    def adjoint(self, lam):
+        """Map lam = dJ/du at the end of the recording, in place, to dJ/du at its start:
+        the start of the last integrate call (rktvd) or of the method (mstvd)."""
+        call("hrw_ode_set_stream", self._h, _stream())
+        call("hrw_ode_adjoint", self._h, _dvec(lam, self.neq, "lambda"))
+        return lam
+
     def set_check(self, check_finite: bool = True):
         """Scan u for NaN/Inf at the end of every integrate call (the default); a hit raises
         HrwError and sets istate = -1."""
@@ -523,9 +538,10 @@ class BurgersBatch:
 
 
 class Adjoint:
-    """Discrete adjoint of ``nsteps`` fixed rktvd steps of periodic Lax-Friedrichs Burgers:
-    one line (a :class:`Burgers1D` problem, any order, step ``dt``) or every line of a
-    :class:`BurgersBatch` (RK3, each line's own dt).  ``forward`` records u0 -> u_N with a
+    """Discrete adjoint of ``nsteps`` fixed rktvd steps of 1D Burgers on a linear grid: one
+    line (a :class:`Burgers1D` problem with either flux and either kind of ends, any order,
+    step ``dt``) or every line of a Lax-Friedrichs :class:`BurgersBatch` (RK3, each line's own
+    dt).  ``forward`` records u0 -> u_N with a
     checkpoint every ``ckpt_every`` steps (0: the largest interval that fits in half of the free
     device memory, usually the whole run, so nothing is recomputed); ``backward`` maps dJ/du_N
     to dJ/du_0.  The reference has no adjoint (this is the added differentiation form of

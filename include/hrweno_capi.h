@@ -289,10 +289,13 @@ int hrw_batch_set_form(hrw_batch_t b, int form);
 int hrw_batch_config(hrw_batch_t b, int *cells_per_lane, int *form);
 
 /* ---------------------------------------------------------------- discrete adjoint - */
-/* Reverse mode of nsteps fixed rktvd steps (src/hrweno_tvdode.f90:136-171) of periodic 1D
- * Burgers with the Lax-Friedrichs flux (example/example1_burgers_1d_fv.f90:72-109 on a
- * periodic line; src/hrweno_fluxes.f90:22-45; src/hrweno_weno.f90:174-216).  The reference
- * has no adjoint; this is the added differentiation form.  forward records u0 -> u_N with a
+/* Reverse mode of nsteps fixed rktvd steps (src/hrweno_tvdode.f90:136-171) of a fused 1D
+ * Burgers problem on a linear grid: the Lax-Friedrichs or the endpoint Godunov flux, periodic
+ * or zero-gradient ends (example/example1_burgers_1d_fv.f90:72-109; src/hrweno_fluxes.f90:
+ * 22-76; src/hrweno_weno.f90:174-216).  The reference has no adjoint; this is the added
+ * differentiation form.  Periodic Lax-Friedrichs lines take the register-blocked reverse
+ * kernel, the other compositions a general reverse pass.  At vm == vp the Godunov flux is
+ * differentiated as the upwind state (where it is smooth).  forward records u0 -> u_N with a
  * checkpoint every ckpt_every steps; backward maps a cotangent dJ/du_N to dJ/du_0,
  * recomputing one segment of ckpt_every steps at a time.  With alpha = max|u| the gradient
  * takes the subgradient at the first cell attaining the maximum.  Device memory:
@@ -301,8 +304,8 @@ int hrw_batch_config(hrw_batch_t b, int *cells_per_lane, int *form);
  * of the run fits (ckpt_every = nsteps), the backward recomputes nothing. */
 int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps, int ckpt_every,
                        hrw_adjoint_t *out);
-/* Every line of a batch (RK3, each line with its own dt); runs on the batch's stream and
- * needs the batch to outlive it.  States are the batch's flat layout. */
+/* Every line of a batch (RK3, each line with its own dt; the Lax-Friedrichs flux); runs on
+ * the batch's stream and needs the batch to outlive it.  States are the batch's flat layout. */
 int hrw_batch_adjoint_create(hrw_batch_t b, int64_t nsteps, int ckpt_every, hrw_adjoint_t *out);
 int hrw_adjoint_destroy(hrw_adjoint_t a);
 int hrw_adjoint_set_stream(hrw_adjoint_t a, hrw_stream_t stream);
@@ -313,10 +316,16 @@ int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN);
 int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g0);
 int hrw_adjoint_info(hrw_adjoint_t a, int64_t *neq, int64_t *nsteps, int *ckpt_every,
                      int64_t *device_bytes);
-/* ode%tape(ckpt_every) / ode%adjoint(lambda): the same reverse mode on an rktvd object built
- * on a fused problem (hrw_rktvd_create).  With ckpt_every > 0 every later hrw_ode_integrate
- * records its steps (0 stops recording); hrw_ode_adjoint maps lambda (device, neq) from
- * dJ/du at the end of the last recorded integrate call to dJ/du at its start, in place. */
+/* ode%tape(ckpt_every) / ode%adjoint(lambda): the same reverse mode on an integrator built on
+ * a fused problem of the kinds above.  rktvd (hrw_rktvd_create): with ckpt_every > 0 every
+ * later hrw_ode_integrate records its steps (0 stops recording); hrw_ode_adjoint maps lambda
+ * (device, neq) from dJ/du at the end of the last recorded integrate call to dJ/du at its
+ * start, in place.  mstvd (hrw_mstvd_create; src/hrweno_tvdode.f90:203-271): the recording
+ * keeps every state from the integrate call that starts the method (istate 1; tape must be set
+ * before it, else that integrate call fails with HRW_ERR_STATE) through every later call --
+ * N + 9 states for N steps -- and hrw_ode_adjoint maps dJ/du at the end of the last call to
+ * dJ/du at the start of the method, the four RK3 starting steps included.  hrw_ode_load and
+ * ckpt_every = 0 drop an mstvd recording. */
 int hrw_ode_tape(hrw_ode_t ode, int ckpt_every);
 int hrw_ode_adjoint(hrw_ode_t ode, double *lambda);
 

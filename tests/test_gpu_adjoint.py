@@ -236,8 +236,11 @@ def test_config5_batch_full_equals_per_line(hh, dev):
 
 
 def test_adjoint_argument_errors(hh):
-    with pytest.raises(hh.HrwError, match="periodic"):
-        hh.Adjoint(hh.Burgers1D(100, bc=hh.BC_ZERO_GRADIENT), nsteps=3, dt=1e-3)
+    # (zero-gradient ends and the Godunov flux take the general reverse pass; a non-uniform
+    # grid's own reconstruction coefficients are not differentiated)
+    e = np.linspace(-1.0, 1.0, 101) ** 3
+    with pytest.raises(hh.HrwError, match="linear grid"):
+        hh.Adjoint(hh.Burgers1D(100, bc=hh.BC_ZERO_GRADIENT, xedges=e), nsteps=3, dt=1e-3)
     with pytest.raises(hh.HrwError, match="ckpt_every"):
         hh.Adjoint(hh.Burgers1D(100), nsteps=3, dt=1e-3, ckpt_every=-1)
     adj = hh.Adjoint(hh.Burgers1D(100), nsteps=3, dt=1e-3)
