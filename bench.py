@@ -361,12 +361,18 @@ def run_adjoint_general(args, torch, hh, W, dev):
     reverse pass: the rktvd gradient of 0.5*|u_N - target|^2 (every stage input kept, so the
     backward is the reverse launches alone) and the recorded mstvd integrator's (hrw_ode_tape /
     hrw_ode_adjoint, the four RK3 starting steps included), each against its plain forward."""
-    c, noise = W.config5(steps=args.steps)
+    import numpy as np
+    c, _ = W.config5(steps=args.steps)
     n, steps, dt = c.n, c.steps, c.dt
     prob = hh.Burgers1D(n, flux=hh.GODUNOV, bc=hh.BC_ZERO_GRADIENT)
     adj = hh.Adjoint(prob, nsteps=steps, dt=dt, ckpt_every=steps)
     u0 = torch.as_tensor(c.u0, device=dev).clone()
-    target = adj.forward(torch.as_tensor(c.u0 + noise, device=dev)).clone()
+    # the target from a smooth perturbation: with per-cell noise the endpoint Godunov flux
+    # turns unstable at the transonic point of this state within 400 steps (the reference's
+    # arithmetic too; the oracle shows it at 2^18 cells)
+    _, xc, _ = W.linear_centres(-1.0, 1.0, n)
+    pert = 0.01 * np.sin(3.0 * np.pi * xc)
+    target = adj.forward(torch.as_tensor(c.u0 + pert, device=dev)).clone()
     res = {}
     adj.backward(adj.forward(u0) - target)          # warm-up
     ms_f = timed(torch, lambda: res.__setitem__("uN", adj.forward(u0)))
@@ -380,23 +386,25 @@ def run_adjoint_general(args, torch, hh, W, dev):
     ms_p = timed(torch, lambda: ode.integrate(up, 0.0, tout, dt))
     del adj
 
-    # mstvd: a recording starts with the method, so each recorded run is a fresh object
-    def ms_record():
-        o = hh.MsTvd(prob).tape(1)
-        u = u0.clone()
-        o.integrate(u, 0.0, tout, dt)
-        res["ms"] = (o, u)
-    ms_record()                                      # warm-up
-    ms_mf = timed(torch, ms_record)
-    o, um = res["ms"]
+    # mstvd: a recording starts with the method (istate 1); loading the state saved before the
+    # first step starts it again, and the recording reuses the storage of the previous one
+    om = hh.MsTvd(prob).tape(1)
+    um = u0.clone()
+    s0 = om.save(um, 0.0)
+    om.integrate(um, 0.0, tout, dt)                  # warm-up (allocates the tape)
+    om.load(s0, um)
+    ms_mf = timed(torch, lambda: om.integrate(um, 0.0, tout, dt))
     lam = um - target
-    ms_mb = timed(torch, lambda: o.adjoint(lam))
+    ms_mb = timed(torch, lambda: om.adjoint(lam))
     if not bool(torch.isfinite(lam).all()):
         raise RuntimeError("non-finite gradient")
-    del res["ms"], o
-    om = hh.MsTvd(prob)
+    del om
+    op = hh.MsTvd(prob)
     up = u0.clone()
-    ms_mp = timed(torch, lambda: om.integrate(up, 0.0, tout, dt))
+    p0 = op.save(up, 0.0)
+    op.integrate(up, 0.0, tout, dt)                  # warm-up
+    op.load(p0, up)
+    ms_mp = timed(torch, lambda: op.integrate(up, 0.0, tout, dt))
     log(f"adjoint (example 1 composition): rktvd forward(recording) {ms_f:.2f} ms, backward "
         f"{ms_b:.2f} ms, plain forward {ms_p:.2f} ms; mstvd recording {ms_mf:.2f} ms, adjoint "
         f"{ms_mb:.2f} ms, plain forward {ms_mp:.2f} ms")
@@ -404,14 +412,14 @@ def run_adjoint_general(args, torch, hh, W, dev):
                   "hrw::rhs_vjp_kernel<3,GODUNOV,zero-gradient,fixed>", "adjoint_general_2p20")
     config = {"workload": "burgers1d_example1_godunov_zerograd_weno5_adjoint_2^20cells_400steps",
               "cells": n, "steps": steps, "k": 3, "eps": 1e-6,
-              "loss": "0.5*|u_N - F(u0 + N(0,0.01))|^2", "parallelism": "single GPU",
+              "loss": "0.5*|u_N - F(u0 + 0.01 sin(3 pi x))|^2", "parallelism": "single GPU",
               "value_is": "cell-steps of the rktvd gradient (recording forward + backward) per s",
               "gradient_over_forward": (ms_f + ms_b) / ms_p,
               "backward_over_forward": ms_b / ms_p,
               "mstvd_gradient_over_forward": (ms_mf + ms_mb) / ms_mp,
               "mstvd_recording_ms": ms_mf, "mstvd_adjoint_ms": ms_mb, "mstvd_forward_ms": ms_mp}
     return n * steps / ((ms_f + ms_b) / 1e3), (ms_f + ms_b) / steps, rl, config, \
-        "synthetic (seeded SplitMix64, config-1 style data; target from N(0,0.01) noise)"
+        "synthetic (seeded SplitMix64, config-1 style data; target from a smooth perturbation)"
 
 
 def run_adjoint_batch(args, torch, hh, W, dev):

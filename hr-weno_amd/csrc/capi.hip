@@ -1065,7 +1065,9 @@ struct hrw_ode_s {
     // inputs of each of the four RK3 starting steps, each step's dt; the reverse sweep's
     // cotangent ring (6) and stage cotangents (2), and the general pass's scratch
     std::vector<double *> ms_tape;
-    std::vector<double *> ms_blocks;       // the tape's storage: one block per recorded call
+    // the tape's storage, one block (pointer, states) per recorded call; the blocks of a
+    // recording that ended are kept for the next one (ms_pool)
+    std::vector<std::pair<double *, int64_t>> ms_blocks, ms_pool;
     double *ms_next = nullptr;             // next free state of the last block
     int64_t ms_free = 0;                   // states left in it
     double *ms_start[8] = {};
@@ -1076,9 +1078,17 @@ struct hrw_ode_s {
     unsigned long long *d_bad = nullptr;   // the scan's result word
     ~hrw_ode_s();
 };
-static void ms_tape_clear(hrw_ode_s *o)
+// Drop the recording; keep_storage: its blocks go to the pool for the next recording.
+static void ms_tape_clear(hrw_ode_s *o, bool keep_storage = false)
 {
-    for (auto &b : o->ms_blocks) dfree(b);
+    for (auto &b : o->ms_blocks) {
+        if (keep_storage) o->ms_pool.push_back(b);
+        else dfree(b.first);
+    }
+    if (!keep_storage) {
+        for (auto &b : o->ms_pool) dfree(b.first);
+        o->ms_pool.clear();
+    }
     o->ms_blocks.clear();
     o->ms_tape.clear();
     o->ms_dts.clear();
@@ -1283,11 +1293,22 @@ static int rk_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
 static int ms_reserve(hrw_ode_s *o, int64_t nstates)
 {
     if (nstates <= 0) return HRW_OK;
-    double *b = nullptr;
-    CALL(dalloc(&b, (size_t)nstates * (size_t)o->neq));
-    o->ms_blocks.push_back(b);
-    o->ms_next = b;
-    o->ms_free = nstates;
+    // the smallest pooled block that holds them, else a new one
+    size_t best = o->ms_pool.size();
+    for (size_t i = 0; i < o->ms_pool.size(); ++i)
+        if (o->ms_pool[i].second >= nstates &&
+            (best == o->ms_pool.size() || o->ms_pool[i].second < o->ms_pool[best].second))
+            best = i;
+    std::pair<double *, int64_t> blk{nullptr, nstates};
+    if (best < o->ms_pool.size()) {
+        blk = o->ms_pool[best];
+        o->ms_pool.erase(o->ms_pool.begin() + (std::ptrdiff_t)best);
+    } else {
+        CALL(dalloc(&blk.first, (size_t)nstates * (size_t)o->neq));
+    }
+    o->ms_blocks.push_back(blk);
+    o->ms_next = blk.first;
+    o->ms_free = blk.second;
     return HRW_OK;
 }
 static int ms_record(hrw_ode_s *o, const double *src)
@@ -1308,7 +1329,7 @@ static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
     // recording (hrw_ode_tape): the tape holds every state since the method started
     const bool rec = o->tape_every > 0;
     if (rec) {
-        if (o->istate == 1) ms_tape_clear(o);
+        if (o->istate == 1) ms_tape_clear(o, true);
         else if (o->ms_tape.empty())
             return fail(HRW_ERR_STATE, "recording a multistep integration starts with the method: "
                                        "enable tape before the first integrate call (istate = 1)");
@@ -1522,7 +1543,7 @@ extern "C" int hrw_ode_load(hrw_ode_t ode, double *u, double *t, const void *buf
     }
     HIP_TRY(hipStreamSynchronize(ode->stream));
     ode->fevals = h.fevals;
-    ms_tape_clear(ode);   // recorded states of another trajectory
+    ms_tape_clear(ode, true);   // recorded states of another trajectory
     // a multistep state saved before its history existed starts over (istate 1)
     ode->istate = ode->method == METHOD_MS && !h.history && h.istate >= 2 ? 1 : h.istate;
     *t = h.t;

@@ -16,7 +16,7 @@
 // interfaces exist; window positions past an end are the end cell (constant extrapolation,
 // src/hrweno_weno.f90:171-173), so their cotangents land on it.
 //
-// Layout: block b takes the output cells [256b, 256b + 256).  It loads the stage input over
+// Layout: block b takes the output cells [Tb, Tb + T) (T = kGenTile).  It loads the stage input over
 // the window halo into LDS, reconstructs the T + 2K positions whose faces its interfaces need,
 // forms the window cotangent of the T + 2K - 2 positions whose windows reach its cells, and
 // gathers each output cell's share in a fixed order (so the result does not depend on timing).
@@ -29,11 +29,15 @@
 #include <climits>
 
 #include "kernels_1d.hpp"
+#include "kernels_1d_fast.hpp"
 #include "weno_math.hpp"
 
 namespace hrw {
 
-constexpr int kGenTile = 256;   // output cells per block (one thread each)
+// Output cells per block (one thread each).  The block also reconstructs the 2K positions of
+// its window halo, which takes one of its waves a second round: with 8 waves that is an
+// eighth of the work on top (a quarter with 4).
+constexpr int kGenTile = 512;
 
 struct GenAdjArgs {
     const double *s;                 // the stage input the Jacobian is taken at
@@ -59,9 +63,10 @@ struct GCell {
 
 // x[0 .. 2K-2] = v_{i-K+1 .. i+K-1}.  The weights over a common denominator,
 //   w_r = d_r Q_r / sum_s d_s Q_s,  Q_r = prod_{s != r} q_s,  q_s = (eps + beta_s)^2,
-// with every q scaled by the largest first (the products stay in (0, 1]): 6 divides per cell
-// instead of the reference order's 12 + 3 (the reverse needs the derivatives at the point to
-// ~1e-15, not the reference's rounding).
+// with every q scaled by the largest first (the products stay in (0, 1]): 6 reciprocals per
+// cell (hardware estimate + two Newton steps, ~1 ulp; every operand is at least eps or d_min
+// times such a product, far from zero) instead of the reference order's 15 divides (the
+// reverse needs the derivatives at the point to ~1e-15, not the reference's rounding).
 template <int K>
 __device__ __forceinline__ void gcell_fwd(const double *x, double eps, GCell<K> &c)
 {
@@ -80,7 +85,7 @@ __device__ __forceinline__ void gcell_fwd(const double *x, double eps, GCell<K> 
         c.q[r] = qr;
         c.l[r] = lr;
         e[r] = eps + beta[r];
-        c.ib[r] = 1.0 / e[r];
+        c.ib[r] = rcp_nr(e[r]);
         q[r] = e[r] * e[r];
         qm = fmax(qm, q[r]);
     }
@@ -88,7 +93,7 @@ __device__ __forceinline__ void gcell_fwd(const double *x, double eps, GCell<K> 
         c.w[0] = 1.0;
         c.wt[0] = 1.0;
     } else {
-        const double iq = 1.0 / qm;
+        const double iq = rcp_nr(qm);
         double qn[K], Q[K];
 #pragma unroll
         for (int r = 0; r < K; ++r) qn[r] = q[r] * iq;
@@ -106,7 +111,7 @@ __device__ __forceinline__ void gcell_fwd(const double *x, double eps, GCell<K> 
             S = fma_(Tab<K>::d[r], Q[r], S);
             St = fma_(Tab<K>::d[K - 1 - r], Q[r], St);
         }
-        const double iS = 1.0 / S, iSt = 1.0 / St;
+        const double iS = rcp_nr(S), iSt = rcp_nr(St);
 #pragma unroll
         for (int r = 0; r < K; ++r) {
             c.w[r] = (Tab<K>::d[r] * Q[r]) * iS;
