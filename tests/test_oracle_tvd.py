@@ -199,6 +199,30 @@ def test_periodic_padding_equals_wrap(oracle):
     assert np.array_equal(u1, u0 + 1e-3 * L)
 
 
+def test_config1_full_run_is_the_reference(oracle, ref):
+    """Config 1 to t = 0.5 (1000 cells, WENO5-LF-RK3, 1,292 steps with the reference's loop
+    control): the oracle's state is bit-identical to the same steps run through the reference's
+    own compiled reconstruct and flux (oracle/_ref)."""
+    from hrweno_hip import workloads as W
+    c = W.config1()
+    u, t, steps, fe = oracle.burgers_rk(c.u0, c.width, tout=c.tout, dt=c.dt)
+    assert steps == 1292 and fe == 3 * steps and t > c.tout
+    assert np.array_equal(u, ref.burgers_rk3(c.u0, c.width, steps, c.dt))
+
+
+@pytest.mark.parametrize("flux,bc", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_mstvd_200_steps_is_the_reference(oracle, ref, flux, bc):
+    """The multistep method (four RK3 starting steps, then 196 steps of eq. 4.26) on config 1's
+    data, either flux, periodic or zero-gradient ends: bit-identical to example 1's composition
+    on the reference's own modules with the integrator lines restated in the harness."""
+    from hrweno_hip import workloads as W
+    c = W.config1()
+    u, t, fe = oracle.burgers_ms(c.u0, c.width, flux=flux, bc=bc,
+                                 tout=W.steps_tout(0.0, c.dt, 200), dt=c.dt)
+    assert fe == 12 + 196
+    assert np.array_equal(u, ref.burgers_ex(c.u0, c.width, 200, c.dt, flux=flux, bc=bc, method=0))
+
+
 def test_pbe_mass_balance(oracle):
     """Zero-flux boundaries (example 2): total mass is conserved by the 2D scheme."""
     n1, n2 = 24, 20
