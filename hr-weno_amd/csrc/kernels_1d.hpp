@@ -64,6 +64,21 @@ __device__ __forceinline__ unsigned long long wave_umax(unsigned long long v)
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Whole-wavefront shifts by one lane by DPP (wave_shr:1 / wave_shl:1: a VALU operand modifier,
+// no LDS round trip): lane l gets v of lane l-1 (shfl_up1) or of lane l+1 (shfl_down1); the lane
+// with no source (0, resp. 63) keeps its own v -- what __shfl_up / __shfl_down by one over 64
+// lanes return.  Every lane of the wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_wave_shift1(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shfl_up1(double v) { return dpp_wave_shift1<0x138>(v); }
+__device__ __forceinline__ double shfl_down1(double v) { return dpp_wave_shift1<0x130>(v); }
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll

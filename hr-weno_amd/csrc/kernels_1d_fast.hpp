@@ -4,7 +4,8 @@
 // algorithmic 16/24/24 bytes per cell).  This kernel cuts the instructions per cell:
 //
 //  * each lane owns R consecutive cells; the K-1... halo comes from the neighbouring lanes by
-//    ds_bpermute shuffles (LDS pipe, no VALU), lanes 0 and 63 load theirs directly;
+//    DPP wave shifts (shfl_up1 / shfl_down1: a VALU operand modifier, no LDS round trip),
+//    lanes 0 and 63 load theirs directly;
 //  * adjacent cells share work (and, for K = 3 with R = 4, adjacent lanes: a lane takes its
 //    left neighbour's last two triples by shuffles instead of forming them again): with
 //    D = v0 - 2v1 + v2 and E = v0 - v2 on the triple
@@ -298,8 +299,8 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         }
 #pragma unroll
         for (int i = 0; i < H; ++i) {
-            v[i] = __shfl_up(v[R + i], 1, 64);            // lane-1's last H cells
-            v[H + R + i] = __shfl_down(v[H + i], 1, 64);  // lane+1's first H cells
+            v[i] = shfl_up1(v[R + i]);            // lane-1's last H cells
+            v[H + R + i] = shfl_down1(v[H + i]);  // lane+1's first H cells
         }
         if constexpr (H > 0) {
             if (lane == 0) {
@@ -435,19 +436,19 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         }
         // ---- T(c0-2) (bC, P2, P1) and T(c0-1) (bB, P1, P0, bC, P2) from lane-1, cells 0 and 1
         Tri3 l2{}, l1{};
-        l2.bC = __shfl_up(t2.bC, 1, 64);
-        l2.P2 = __shfl_up(t2.P2, 1, 64);
-        l2.P1 = __shfl_up(t2.P1, 1, 64);
-        l1.bB = __shfl_up(t1.bB, 1, 64);
-        l1.P1 = __shfl_up(t1.P1, 1, 64);
-        l1.P0 = __shfl_up(t1.P0, 1, 64);
-        l1.bC = __shfl_up(t1.bC, 1, 64);
-        l1.P2 = __shfl_up(t1.P2, 1, 64);
-        const double vr_left = __shfl_up(vr_prev, 1, 64);   // vr of cell c0-1 (lane-1's last)
+        l2.bC = shfl_up1(t2.bC);
+        l2.P2 = shfl_up1(t2.P2);
+        l2.P1 = shfl_up1(t2.P1);
+        l1.bB = shfl_up1(t1.bB);
+        l1.P1 = shfl_up1(t1.P1);
+        l1.P0 = shfl_up1(t1.P0);
+        l1.bC = shfl_up1(t1.bC);
+        l1.P2 = shfl_up1(t1.P2);
+        const double vr_left = shfl_up1(vr_prev);   // vr of cell c0-1 (lane-1's last)
         double vl0, vr0, vl1, vr1;
         recon3<true>(eps12, l2, l1, ta, vl0, vr0);
         recon3<true>(eps12, l1, ta, tb, vl1, vr1);
-        const double vl_right = __shfl_down(vl0, 1, 64);    // vl of cell c0+R (lane+1's first)
+        const double vl_right = shfl_down1(vl0);    // vl of cell c0+R (lane+1's first)
         const double f0 = numflux(vr_left, vl0);    // face c0-1/2
         const double f1 = numflux(vr0, vl1);        // face c0+1/2
         const double f2 = numflux(vr1, vl2);        // face c0+3/2
@@ -534,8 +535,8 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         }
     }
     // ---- the two faces shared with the neighbouring lanes, then cells 0 and R-1
-    const double vr_left = __shfl_up(vr_prev, 1, 64);    // vr of cell c0-1 (lane-1's last)
-    const double vl_right = __shfl_down(vl0, 1, 64);     // vl of cell c0+R (lane+1's first)
+    const double vr_left = shfl_up1(vr_prev);    // vr of cell c0-1 (lane-1's last)
+    const double vl_right = shfl_down1(vl0);     // vl of cell c0+R (lane+1's first)
     {
         const double f0 = numflux(vr_left, vl0);
         double un, L;

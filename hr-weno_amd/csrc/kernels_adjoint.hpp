@@ -21,7 +21,7 @@
 // from c0 = W + (l-1)*R; lanes 0 and 63 are halo lanes whose results are only used through
 // the neighbour exchange.  Each lane scatters the window cotangents of its R reconstructions
 // into R + 2(K-1) accumulators and hands the K-1 cells that spill past each end to the
-// neighbouring lane (ds_bpermute).  The wave-edge lanes load the window and flux-cotangent
+// neighbouring lane (DPP wave shifts, shfl_up1 / shfl_down1).  The wave-edge lanes load the window and flux-cotangent
 // cells beyond them directly (no neighbour to shuffle from), so their spills into lanes 1
 // and 62 are exact for any R >= 2; only their own cells (which miss the outer spills) are
 // dropped.
@@ -370,8 +370,8 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
         if (x2) load_run<R>(x2 + c0, p2v);
 #pragma unroll
         for (int i = 0; i < H; ++i) {
-            v[i] = __shfl_up(v[R + i], 1, 64);
-            v[H + R + i] = __shfl_down(v[H + i], 1, 64);
+            v[i] = shfl_up1(v[R + i]);
+            v[H + R + i] = shfl_down1(v[H + i]);
         }
         // the wave-edge lanes have no neighbour to shuffle from
         if constexpr (H > 0) {
@@ -454,8 +454,8 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     };
 #pragma unroll
     for (int i = 0; i < R; ++i) phi[i + 1] = mu[i] * inv_w(cell[i]);
-    phi[0] = __shfl_up(phi[R], 1, 64);
-    phi[R + 1] = __shfl_down(phi[1], 1, 64);
+    phi[0] = shfl_up1(phi[R]);
+    phi[R + 1] = shfl_down1(phi[1]);
     if (edge) {
         const double ph = lam_edge * inv_w(hc);
         if (lane == 0) phi[0] = ph;
@@ -505,8 +505,8 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
     // hand the spill-over cells to the neighbours: acc[j] <-> cell c0 - H + j
 #pragma unroll
     for (int i = 0; i < H; ++i) {
-        const double fromR = __shfl_down(acc[i], 1, 64);           // right lane's cells c0+R+i-H...
-        const double fromL = __shfl_up(acc[R + H + i], 1, 64);
+        const double fromR = shfl_down1(acc[i]);           // right lane's cells c0+R+i-H...
+        const double fromL = shfl_up1(acc[R + H + i]);
         acc[R + i] += fromR;
         acc[H + i] += fromL;
     }
