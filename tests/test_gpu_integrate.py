@@ -249,6 +249,27 @@ def test_fast_mstvd_and_rhs(oracle, hh, dev, r):
     assert np.max(np.abs(L - oL)) <= 1e-12 * np.abs(oL).max()
 
 
+@pytest.mark.parametrize("k", [2, 3])
+def test_persistent_form_rhs_and_mstvd_identical(hh, dev, k):
+    """The persistent form (with 4 cells per lane: two chunks of loads in flight per wave)
+    against one wavefront per chunk, bit for bit, in its three store patterns: an rhs-only pass
+    (stores L), the multistep integrator (its starting RK3 stages store u; its multistep stages
+    store u and L) on lengths around the first and the last interior chunk."""
+    from hrweno_hip import workloads as W
+    for n in (255, 256, 1000, 20000, 300001):
+        c = W.config1(n=n, seed=30 + n)
+        out = {}
+        for variant in (0, 2):
+            p = hh.Burgers1D(n, k, 1e-6, arith="fast").tune(variant, 4)
+            L = _h(p.rhs(0.0, _t(c.u0, dev)))
+            ode = hh.MsTvd(p)
+            u = _t(c.u0, dev)
+            ode.integrate(u, 0.0, W.steps_tout(0.0, c.dt, 8), c.dt)
+            out[variant] = (L, _h(u))
+        assert np.array_equal(out[0][0], out[2][0]), n
+        assert np.array_equal(out[0][1], out[2][1]), n
+
+
 def test_mstvd_resume_across_calls(oracle, hh, dev):
     """History persists across integrate calls (istate 2), as in the reference."""
     from hrweno_hip import workloads as W
