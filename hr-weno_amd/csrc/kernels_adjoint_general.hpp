@@ -35,9 +35,12 @@
 namespace hrw {
 
 // Output cells per block (one thread each).  The block also reconstructs the 2K positions of
-// its window halo, which takes one of its waves a second round: with 8 waves that is an
-// eighth of the work on top (a quarter with 4).
-constexpr int kGenTile = 512;
+// its window halo, which takes one of its waves a second round: a quarter of the work on top
+// with 4 waves, an eighth with 8.  The pass is bound by the latency of its barrier-separated
+// phases, and 4-wave blocks (five resident per CU at 92-94 VGPRs, against two of 8 waves)
+// overlap one block's waits with another's work: 4 % faster at 2^20 cells than 8-wave blocks
+// despite the halo, and 2-wave blocks lose it again to the halo (11 % slower).
+constexpr int kGenTile = 256;
 
 struct GenAdjArgs {
     const double *s;                 // the stage input the Jacobian is taken at
