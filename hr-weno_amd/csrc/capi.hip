@@ -2085,20 +2085,41 @@ struct hrw_adjoint_s {
     unsigned tag = 0;
     int64_t launches = 0;
     std::vector<std::pair<const double *, Pending>> owed;   // buffer -> its pending correction
-    int64_t y_seg = -1;           // segment whose stage inputs Y and the sets hold
+    // Stage-input buffers: 0 = Y, sets, alpha_rec; 1 = Y1, sets1, alpha_rec1 (one line with more
+    // than one segment, allocated by the first backward that can use it): the backward then
+    // recomputes the next segment into one buffer on the second stream `side` while it reverses
+    // the current segment from the other.  buf_seg[b]: the segment buffer b holds (-1: none).
+    std::vector<double *> Y1;
+    unsigned long long *sets1 = nullptr;
+    double *alpha_rec1 = nullptr;
+    double *sink = nullptr;       // a recompute's last output (the checkpoint, whose copy is read)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_a = nullptr, ev_fill[2] = {nullptr, nullptr};
+    bool two_buffers = false, no_two_buffers = false;
+    int64_t buf_seg[2] = {-1, -1};
     bool recorded = false;
     bool general = false;         // one line that takes the general reverse pass
     GenScratch gen;
     hipStream_t stream = nullptr;
     int nlines() const { return b ? b->nlines : 1; }
+    std::vector<double *> &ybuf(int k) { return k ? Y1 : Y; }
+    unsigned long long *setsbuf(int k) const { return k ? sets1 : sets; }
+    double *arec(int k) const { return k ? alpha_rec1 : alpha_rec; }
     hipStream_t strm() const { return b ? b->stream : stream; }
     bool alphamax() const { return b ? b->alphamax() : p->alphamax(); }
     int64_t seg_len(int64_t s) const { return std::min<int64_t>(m, nsteps - s * m); }
     ~hrw_adjoint_s()
     {
+        if (side) (void)hipStreamSynchronize(side);
         for (auto &c : ckpt) dfree(c);
         for (auto &y : Y) dfree(y);
+        for (auto &y : Y1) dfree(y);
         dfree(lamA); dfree(lamB); dfree(sets); dfree(alpha_rec);
+        dfree(sets1); dfree(alpha_rec1); dfree(sink);
+        if (ev_a) (void)hipEventDestroy(ev_a);
+        for (auto &e : ev_fill)
+            if (e) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
         if (own_lines) {
             dfree(d_off); dfree(d_n); dfree(d_k); dfree(d_h); dfree(d_xmin); dfree(d_rx);
         }
@@ -2365,50 +2386,54 @@ extern "C" int hrw_adjoint_info(hrw_adjoint_t a, int64_t *neq, int64_t *nsteps, 
     if (nsteps) *nsteps = a->nsteps;
     if (ckpt_every) *ckpt_every = a->m;
     if (device_bytes)
-        *device_bytes = (int64_t)((a->ckpt.size() + a->Y.size() + 2) * (size_t)a->flat * sizeof(double) +
-                                  a->set_words * (size_t)(a->order * a->m + 2) * 8);
+        *device_bytes = (int64_t)((a->ckpt.size() + a->Y.size() + 2 + (a->two_buffers ? a->Y1.size() + 1 : 0)) *
+                                      (size_t)a->flat * sizeof(double) +
+                                  a->set_words * (size_t)(a->order * a->m + 2) * 8 * (a->two_buffers ? 2 : 1));
     return HRW_OK;
 }
 
-static int adjoint_seed(hrw_adjoint_s *a, const double *u)
+static int adjoint_seed(hrw_adjoint_s *a, const double *u, int k, hipStream_t s)
 {
     if (!a->alphamax()) return HRW_OK;
-    hipStream_t s = a->strm();
+    unsigned long long *sets = a->setsbuf(k);
     if (a->b) {
         hrw_batch_s *b = a->b;
-        fill_u64_kernel<<<grid_for(2LL * b->nlines, 256, 4096), 256, 0, s>>>(a->sets, 2LL * b->nlines);
+        fill_u64_kernel<<<grid_for(2LL * b->nlines, 256, 4096), 256, 0, s>>>(sets, 2LL * b->nlines);
         batch_maxabs_kernel<<<(unsigned)((b->nchunks * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-            u, b->d_off, b->d_n, b->d_cline, b->d_cstart, b->nchunks, b->chunk_R, a->sets);
+            u, b->d_off, b->d_n, b->d_cline, b->d_cstart, b->nchunks, b->chunk_R, sets);
     } else {
-        clear_slots_kernel<<<grid_for(2 * kNSlot, 256), 256, 0, s>>>(a->sets, 2);
-        maxabs_kernel<<<grid_for(a->p->neq, kBlock * 8, 2048), kBlock, 0, s>>>(u, a->p->neq, a->sets);
+        clear_slots_kernel<<<grid_for(2 * kNSlot, 256), 256, 0, s>>>(sets, 2);
+        maxabs_kernel<<<grid_for(a->p->neq, kBlock * 8, 2048), kBlock, 0, s>>>(u, a->p->neq, sets);
     }
     LAUNCH_CHECK();
     return HRW_OK;
 }
 
-static double *seg_state(hrw_adjoint_s *a, int64_t s, int64_t q)
+// Stage input q of segment s, its inner ones from buffer k.
+static double *seg_state(hrw_adjoint_s *a, int64_t s, int64_t q, int k = 0)
 {
     const int64_t last = a->order * a->seg_len(s);
     if (q == 0) return a->ckpt[(size_t)s];
     if (q == last) return a->ckpt[(size_t)s + 1];
-    return a->Y[(size_t)q - 1];
+    return a->ybuf(k)[(size_t)q - 1];
 }
 
-// Forward over segment s from its checkpoint; `full` = also the last launch (whose output is
-// the next checkpoint).
-static int adjoint_segment(hrw_adjoint_s *a, int64_t s, bool full)
+// Forward over segment s from its checkpoint into buffer k on stream st; `full` = also the last
+// launch (whose output is the next checkpoint, or last_out when given: a recompute running beside
+// the reverse of the next segment, which reads that checkpoint, writes its copy elsewhere).
+static int adjoint_segment(hrw_adjoint_s *a, int64_t s, bool full, int k, hipStream_t st,
+                           double *last_out = nullptr)
 {
     static const int stage_id[3][3] = {{RK1, 0, 0}, {RK2_1, RK2_2, 0}, {RK3_1, RK3_2, RK3_3}};
     const int64_t nq = a->order * a->seg_len(s) - (full ? 0 : 1);
-    CALL(adjoint_seed(a, seg_state(a, s, 0)));
+    CALL(adjoint_seed(a, seg_state(a, s, 0, k), k, st));
     const size_t W = a->set_words;
     for (int64_t q = 0; q < nq; ++q) {
-        const int st = (int)(q % a->order);
-        const double *us = seg_state(a, s, q), *u0 = seg_state(a, s, q - st);
-        double *uo = seg_state(a, s, q + 1);
-        unsigned long long *ain = a->sets + (size_t)q * W, *aout = ain + W, *aclr = aout + W;
-        const int stage = stage_id[a->order - 1][st];
+        const int st_i = (int)(q % a->order);
+        const double *us = seg_state(a, s, q, k), *u0 = seg_state(a, s, q - st_i, k);
+        double *uo = q + 1 == a->order * a->seg_len(s) && last_out ? last_out : seg_state(a, s, q + 1, k);
+        unsigned long long *ain = a->setsbuf(k) + (size_t)q * W, *aout = ain + W, *aclr = aout + W;
+        const int stage = stage_id[a->order - 1][st_i];
         if (a->b) {
             CALL(batch_stage_sets(a->b, ain, aout, aclr, stage, us, u0, uo));
         } else {
@@ -2416,11 +2441,46 @@ static int adjoint_segment(hrw_adjoint_s *a, int64_t s, bool full)
             io.us = us;
             io.u0 = u0;
             io.uo = uo;
-            io.alpha_record = a->alpha_rec + q;
-            CALL(run_stage_sets(a->p, ain, aout, aclr, stage, a->dt, io, a->stream));
+            io.alpha_record = a->arec(k) + q;
+            CALL(run_stage_sets(a->p, ain, aout, aclr, stage, a->dt, io, st));
         }
     }
     return HRW_OK;
+}
+
+// The second stage-input buffer, its stream and events (one line, more than one segment); false
+// when the device has no room for it (the backward then recomputes and reverses in turn).
+static bool adjoint_two_buffers(hrw_adjoint_s *a)
+{
+    if (a->two_buffers) return true;
+    if (a->b || a->nseg < 2 || a->no_two_buffers) return false;
+    bool ok = true;
+    a->Y1.assign(a->Y.size(), nullptr);
+    for (auto &y : a->Y1) ok = ok && dalloc(&y, (size_t)a->flat) == HRW_OK;
+    ok = ok && dalloc(&a->sets1, a->set_words * (size_t)(a->order * a->m + 2)) == HRW_OK;
+    ok = ok && dalloc(&a->alpha_rec1, (size_t)(a->order * a->m + 1)) == HRW_OK;
+    ok = ok && dalloc(&a->sink, (size_t)a->flat) == HRW_OK;
+    ok = ok && hipStreamCreateWithFlags(&a->side, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&a->ev_a, hipEventDisableTiming) == hipSuccess;
+    for (auto &e : a->ev_fill) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {   // no room: free what was taken, and do not try again
+        (void)hipGetLastError();
+        for (auto &y : a->Y1) dfree(y);
+        a->Y1.clear();
+        dfree(a->sets1); dfree(a->alpha_rec1); dfree(a->sink);
+        if (a->ev_a) (void)hipEventDestroy(a->ev_a);
+        for (auto &e : a->ev_fill)
+            if (e) (void)hipEventDestroy(e);
+        if (a->side) (void)hipStreamDestroy(a->side);
+        a->ev_a = nullptr;
+        a->ev_fill[0] = a->ev_fill[1] = nullptr;
+        a->side = nullptr;
+        a->no_two_buffers = true;
+        return false;
+    }
+    a->two_buffers = true;
+    a->buf_seg[1] = -1;
+    return true;
 }
 
 extern "C" int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN)
@@ -2430,25 +2490,28 @@ extern "C" int hrw_adjoint_forward(hrw_adjoint_t a, const double *u0, double *uN
     const size_t bytes = (size_t)a->flat * sizeof(double);
     hipStream_t s = a->strm();
     std::unique_ptr<BatchCall> order(a->b ? new BatchCall(a->b) : nullptr);
+    if (a->side) HIP_TRY(hipStreamSynchronize(a->side));   // (idle unless a call failed midway)
     HIP_TRY(hipMemcpyAsync(a->ckpt[0], u0, bytes, hipMemcpyDeviceToDevice, s));
     a->recorded = false;
-    for (int64_t g = 0; g < a->nseg; ++g) CALL(adjoint_segment(a, g, true));
-    a->y_seg = a->nseg - 1;
+    for (int64_t g = 0; g < a->nseg; ++g) CALL(adjoint_segment(a, g, true, 0, s));
+    a->buf_seg[0] = a->nseg - 1;
+    a->buf_seg[1] = -1;
     HIP_TRY(hipMemcpyAsync(uN, a->ckpt[(size_t)a->nseg], bytes, hipMemcpyDeviceToDevice, s));
     a->recorded = true;
     return HRW_OK;
 }
 
-// out = ca*lam + cc*h*J(X[q])^T lam + e1*x1 + e2*x2 over every line.
-static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *lam, double *out,
-                         double ca, double cc, const double *x1, double e1, const double *x2,
-                         double e2)
+// out = ca*lam + cc*h*J(X[q])^T lam + e1*x1 + e2*x2 over every line (X[q] of segment s from
+// buffer k).
+static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, int k, const double *lam,
+                         double *out, double ca, double cc, const double *x1, double e1,
+                         const double *x2, double e2)
 {
     if (a->general)   // (its alpha-bar correction is applied within the pass: nothing owed)
-        return gen_vjp(a->p, a->gen, a->strm(), seg_state(a, s, q), lam, 1.0, nullptr, 0.0,
+        return gen_vjp(a->p, a->gen, a->strm(), seg_state(a, s, q, k), lam, 1.0, nullptr, 0.0,
                        {{lam, ca}, {x1, e1}, {x2, e2}}, out, cc * a->dt);
     AdjArgs g{};
-    g.s = seg_state(a, s, q);
+    g.s = seg_state(a, s, q, k);
     g.lam = lam;
     g.x1 = x1;
     g.x2 = x2;
@@ -2465,7 +2528,7 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, const double *l
     g.anpart = 1;
     g.apart_stride = 0;
     g.aset = a->b ? a->sets + (size_t)q * a->set_words
-                  : reinterpret_cast<const unsigned long long *>(a->alpha_rec + q);
+                  : reinterpret_cast<const unsigned long long *>(a->arec(k) + q);
     const bool amax = a->alphamax();
     if (amax) {
         if (a->tag >= kNoPendTag - 16) {   // tags wrap: start over from clean slots
@@ -2518,28 +2581,50 @@ extern "C" int hrw_adjoint_backward(hrw_adjoint_t a, const double *gN, double *g
     std::unique_ptr<BatchCall> order(a->b ? new BatchCall(a->b) : nullptr);
     double *lam = g0, *A = a->lamA, *B = a->lamB;
     a->owed.clear();
+    if (a->side) HIP_TRY(hipStreamSynchronize(a->side));   // (idle unless a call failed midway)
     if (gN != g0) HIP_TRY(hipMemcpyAsync(g0, gN, bytes, hipMemcpyDeviceToDevice, st));
+    // With two stage-input buffers the segments' recomputation overlaps the reverse: while
+    // segment s is reversed from buffer k on st, segment s-1 is recomputed into the other on the
+    // side stream, after st's launches that read that buffer (those of segment s+1).
+    const bool two = adjoint_two_buffers(a);
+    bool filling[2] = {false, false};   // buffer filled on the side stream, not yet waited for
     for (int64_t s = a->nseg - 1; s >= 0; --s) {
-        if (a->y_seg != s) {
-            // the whole segment again (its last launch rewrites ckpt[s+1] with the same
-            // values, and records the alpha of the last stage input)
-            CALL(adjoint_segment(a, s, true));
-            a->y_seg = s;
+        int k = a->buf_seg[0] == s ? 0 : (a->buf_seg[1] == s ? 1 : -1);
+        if (k < 0) {
+            // the whole segment again, in turn (its last launch rewrites ckpt[s+1] with the
+            // same values, and records the alpha of the last stage input)
+            k = 0;
+            CALL(adjoint_segment(a, s, true, k, st));
+            a->buf_seg[k] = s;
+        }
+        if (filling[k]) {
+            HIP_TRY(hipStreamWaitEvent(st, a->ev_fill[k], 0));
+            filling[k] = false;
+        }
+        if (two && s >= 1 && a->buf_seg[1 - k] != s - 1) {
+            const int o = 1 - k;
+            HIP_TRY(hipEventRecord(a->ev_a, st));
+            HIP_TRY(hipStreamWaitEvent(a->side, a->ev_a, 0));
+            // (its last output, ckpt[s] again, goes to the sink: this segment's reverse reads ckpt[s])
+            CALL(adjoint_segment(a, s - 1, true, o, a->side, a->sink));
+            HIP_TRY(hipEventRecord(a->ev_fill[o], a->side));
+            a->buf_seg[o] = s - 1;
+            filling[o] = true;
         }
         for (int64_t j = a->seg_len(s) - 1; j >= 0; --j) {
             const int64_t q = a->order * j;
             switch (a->order) {
             case 3:   // u' = (u + 2 u2 + 2h L(u2))/3, u2 = (3u + u1 + h L(u1))/4, u1 = u + h L(u)
-                CALL(adjoint_stage(a, s, q + 2, lam, A, 2.0 / 3.0, 2.0 / 3.0, nullptr, 0, nullptr, 0));
-                CALL(adjoint_stage(a, s, q + 1, A, B, 0.25, 0.25, nullptr, 0, nullptr, 0));
-                CALL(adjoint_stage(a, s, q, B, lam, 1.0, 1.0, A, 0.75, lam, 1.0 / 3.0));
+                CALL(adjoint_stage(a, s, q + 2, k, lam, A, 2.0 / 3.0, 2.0 / 3.0, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q + 1, k, A, B, 0.25, 0.25, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q, k, B, lam, 1.0, 1.0, A, 0.75, lam, 1.0 / 3.0));
                 break;
             case 2:   // u' = (u + u1 + h L(u1))/2, u1 = u + h L(u)
-                CALL(adjoint_stage(a, s, q + 1, lam, A, 0.5, 0.5, nullptr, 0, nullptr, 0));
-                CALL(adjoint_stage(a, s, q, A, lam, 1.0, 1.0, lam, 0.5, nullptr, 0));
+                CALL(adjoint_stage(a, s, q + 1, k, lam, A, 0.5, 0.5, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q, k, A, lam, 1.0, 1.0, lam, 0.5, nullptr, 0));
                 break;
             default:  // u' = u + h L(u)
-                CALL(adjoint_stage(a, s, q, lam, A, 1.0, 1.0, nullptr, 0, nullptr, 0));
+                CALL(adjoint_stage(a, s, q, k, lam, A, 1.0, 1.0, nullptr, 0, nullptr, 0));
                 std::swap(lam, A);
                 break;
             }

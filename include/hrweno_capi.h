@@ -301,7 +301,11 @@ int hrw_batch_config(hrw_batch_t b, int *cells_per_lane, int *form);
  * takes the subgradient at the first cell attaining the maximum.  Device memory:
  * (nsteps/ckpt_every + 1) + (order*ckpt_every - 1) + 2 states.  ckpt_every = 0 chooses the
  * largest interval whose states fit in half of the free device memory; when every stage input
- * of the run fits (ckpt_every = nsteps), the backward recomputes nothing. */
+ * of the run fits (ckpt_every = nsteps), the backward recomputes nothing.  With more than one
+ * segment, a single line's first backward also takes order*ckpt_every states more when they
+ * fit.  Every later backward then recomputes the next segment on a second stream of its own
+ * while it reverses the current one, with the same results (its results are ordered on the
+ * adjoint's stream as always). */
 int hrw_adjoint_create(hrw_problem_t p, int order, double dt, int64_t nsteps, int ckpt_every,
                        hrw_adjoint_t *out);
 /* Every line of a batch (RK3, each line with its own dt; the Lax-Friedrichs flux); runs on
