@@ -1289,7 +1289,10 @@ static int rk_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
 }
 
 // Recording of mstvd%integrate for the adjoint: storage for the states a call adds (one
-// allocation per call, its step count known up front), and a copy of a new state on the tape.
+// allocation per call, its step count known up front), a copy of a new state on the tape, or a
+// new slot for a stage to write into.  Slots are 16-byte aligned (a stride of an even number of
+// doubles), as the stage kernels' vector stores need.
+static size_t ms_stride(const hrw_ode_s *o) { return ((size_t)o->neq + 1) & ~(size_t)1; }
 static int ms_reserve(hrw_ode_s *o, int64_t nstates)
 {
     if (nstates <= 0) return HRW_OK;
@@ -1304,20 +1307,26 @@ static int ms_reserve(hrw_ode_s *o, int64_t nstates)
         blk = o->ms_pool[best];
         o->ms_pool.erase(o->ms_pool.begin() + (std::ptrdiff_t)best);
     } else {
-        CALL(dalloc(&blk.first, (size_t)nstates * (size_t)o->neq));
+        CALL(dalloc(&blk.first, (size_t)nstates * ms_stride(o)));
     }
     o->ms_blocks.push_back(blk);
     o->ms_next = blk.first;
     o->ms_free = blk.second;
     return HRW_OK;
 }
-static int ms_record(hrw_ode_s *o, const double *src)
+static int ms_take(hrw_ode_s *o, double **slot)
 {
     if (o->ms_free <= 0) return fail(HRW_ERR_STATE, "multistep recording: tape storage exhausted");
-    double *b = o->ms_next;
-    o->ms_next += o->neq;
+    *slot = o->ms_next;
+    o->ms_next += ms_stride(o);
     --o->ms_free;
-    o->ms_tape.push_back(b);
+    o->ms_tape.push_back(*slot);
+    return HRW_OK;
+}
+static int ms_record(hrw_ode_s *o, const double *src)
+{
+    double *b = nullptr;
+    CALL(ms_take(o, &b));
     HIP_TRY(hipMemcpyAsync(b, src, o->neq * sizeof(double), hipMemcpyDeviceToDevice, o->stream));
     return HRW_OK;
 }
@@ -1373,6 +1382,52 @@ static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
         }
         o->fevals = fe;
         o->istate = 2;
+    }
+    if (rec && !o->fu) {
+        // recorded fused steps write u_{n+1} straight onto the tape: S and the columns of u
+        // become views, on the tape or on the object's own five buffers (each of those under
+        // at most one view); at the end (or on an error) the views left on the tape get the
+        // free buffers back, with copies of their states
+        double *own[5] = {o->S, o->ring_u[0], o->ring_u[1], o->ring_u[2], o->ring_u[3]};
+        auto settle = [&]() {
+            double **views[5] = {&o->S, &o->ring_u[0], &o->ring_u[1], &o->ring_u[2], &o->ring_u[3]};
+            bool used[5] = {};
+            for (auto v : views)
+                for (int i = 0; i < 5; ++i)
+                    if (*v == own[i]) used[i] = true;
+            int f = 0;
+            hipError_t e = hipSuccess;
+            for (auto v : views) {
+                bool mine = false;
+                for (int i = 0; i < 5; ++i) mine = mine || *v == own[i];
+                if (mine) continue;
+                while (used[f]) ++f;                   // (as many free buffers as tape views)
+                const hipError_t ec = hipMemcpyAsync(own[f], *v, bytes, hipMemcpyDeviceToDevice, o->stream);
+                if (e == hipSuccess) e = ec;
+                *v = own[f];
+                used[f] = true;
+            }
+            return e;
+        };
+        while (!is_done(*t, tout, dt)) {               // :252-268
+            const int h4 = o->head;
+            double *next = nullptr;
+            int rc = ms_take(o, &next);
+            if (rc == HRW_OK)
+                rc = ode_stage(o, seq, MS3, *t, dt, o->S, o->S, next, o->ring_u[h4], o->ring_L[h4],
+                               o->ring_L[h4]);
+            if (rc != HRW_OK) {
+                (void)settle();
+                return rc;
+            }
+            *t = *t + dt;
+            o->fevals += 1;
+            o->ring_u[h4] = o->S;                      // column 1 <- u_n, state <- u_{n+1}
+            o->S = next;
+            o->head = (h4 + 1) % 4;
+            o->ms_dts.push_back(dt);
+        }
+        HIP_TRY(settle());
     }
     for (;;) {                                         // :252-268
         if (is_done(*t, tout, dt)) break;
