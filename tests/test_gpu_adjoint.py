@@ -64,7 +64,12 @@ def test_checkpoint_interval_is_invisible(hh, dev):
         if ck == 0:   # automatic: the whole run fits, every stage input is kept
             assert adj.info()["ckpt_every"] == steps
         adj.forward(_dev(u0, dev))
+        before = adj.info()["device_bytes"]
         outs.append(adj.backward(gN).cpu().numpy())
+        # with more than one segment the backward takes a second set of stage inputs and
+        # recomputes each segment on a second stream beside the reverse of the next
+        grown = adj.info()["device_bytes"] > before
+        assert grown == (ck < steps and ck != 0), ck
         # a second backward of the same forward (segments recomputed again) is identical
         outs.append(adj.backward(gN).cpu().numpy())
     for o in outs[1:]:
