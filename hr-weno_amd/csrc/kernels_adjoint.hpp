@@ -225,9 +225,10 @@ __device__ __forceinline__ PendAt pend_at(unsigned tag, const LineRec &r, int i)
 }
 
 // Reverse of one K = 3 reconstruction of cell i from its window x[0..4] = v_{i-2..i+2}, for
-// the flux cotangents phi3 = (phi_{i-1}, phi_i, phi_{i+1}).  The triple pieces are rebuilt
-// from the window for each cell (cheaper in registers than carrying them between cells).
-// The window cotangent is added to acc[0..4]; returns the cell's share of alpha-bar.
+// the halved flux cotangents phi3 = (phi_{i-1}, phi_i, phi_{i+1}), phi = lam/(2w) (the 1/2 of
+// the Lax-Friedrichs flux folded in: an exact scaling).  The triple pieces are rebuilt from the
+// window for each cell (cheaper in registers than carrying them between cells).  The window
+// cotangent is added to acc[0..4]; returns the cell's share of alpha-bar.
 __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, const double *phi3,
                                              double alpha, double *acc)
 {
@@ -238,9 +239,13 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
     const double D1 = fma_(-2.0, x[2], x[1]) + x[3], E1 = x[1] - x[3];
     const double D0 = fma_(-2.0, x[3], x[2]) + x[4], E0 = x[2] - x[4];
     const double Y2 = fma_(2.0, D2, -E2), X0 = fma_(2.0, D0, E0);
-    const double p2 = eps12 + fma_(3.0 * Y2, Y2, (13.0 * D2) * D2);
-    const double p1 = eps12 + fma_(3.0 * E1, E1, (13.0 * D1) * D1);
-    const double p0 = eps12 + fma_(3.0 * X0, X0, (13.0 * D0) * D0);
+    // (13 D and 3 Y, 3 E, 3 X are kept for the reverse: 26 D pb = 13 D (2 pb) and 6 Y pb =
+    // 3 Y (2 pb) exactly, with 2 pb formed directly below)
+    const double t2 = 13.0 * D2, t1 = 13.0 * D1, t0 = 13.0 * D0;
+    const double y2 = 3.0 * Y2, e1 = 3.0 * E1, x0 = 3.0 * X0;
+    const double p2 = eps12 + fma_(y2, Y2, t2 * D2);
+    const double p1 = eps12 + fma_(e1, E1, t1 * D1);
+    const double p0 = eps12 + fma_(x0, X0, t0 * D0);
     // 6x candidates: right P0(T(i)), P1(T(i-1)), P2(T(i-2)); left Pm(T(i)), P0(T(i-1)), P1(T(i-2))
     const double cr0 = fma_(5.0, x[3], fma_(2.0, x[2], -x[4]));
     const double cr1 = fma_(5.0, x[2], fma_(2.0, x[3], -x[1]));
@@ -256,16 +261,16 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
     const double vr = fma_(a0, cr0, fma_(a1, cr1, m2 * cr2)) * ir;
     const double vl = fma_(m0, cl0, fma_(a1, cl1, l2 * cl2)) * il;
     const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
-    const double sgr = (0.5 * Gp * (vr + alpha)) * ir, sgl = (0.5 * Gm * (vl - alpha)) * il;
+    const double sgr = (Gp * (vr + alpha)) * ir, sgl = (Gm * (vl - alpha)) * il;
     const double vr6 = 6.0 * vr, vl6 = 6.0 * vl;
-    // cotangent of p_r = 12 (eps + beta_r):  -2/p_r [ w-numerator_r sgr (cr_r - 6vr) + ... ]
-    const double ipi = -2.0 * rcp_nr(p0 * p1 * p2);
+    // twice the cotangent of p_r = 12 (eps + beta_r):  -4/p_r [ w-numerator_r sgr (cr_r - 6vr) + ... ]
+    const double ipi = -4.0 * rcp_nr(p0 * p1 * p2);
     const double pb0 = (p1 * p2 * ipi) * fma_(a0 * sgr, cr0 - vr6, (m0 * sgl) * (cl0 - vl6));
     const double pb1 = (p0 * p2 * ipi) * fma_(a1 * sgr, cr1 - vr6, (a1 * sgl) * (cl1 - vl6));
     const double pb2 = (p0 * p1 * ipi) * fma_(m2 * sgr, cr2 - vr6, (l2 * sgl) * (cl2 - vl6));
     // T(i-2): beta_2 (D, Y = 2D - E), candidates P2 (right, m2) and P1 (left, l2)
     {
-        const double Db = 26.0 * D2 * pb2, Yb = 6.0 * Y2 * pb2;
+        const double Db = t2 * pb2, Yb = y2 * pb2;
         const double Pr = m2 * sgr, Pl = l2 * sgl;
         acc[0] += fma_(2.0, Pr, Db + Yb - Pl);
         acc[1] += fma_(-7.0, Pr, fma_(5.0, Pl, fma_(-4.0, Yb, -2.0 * Db)));
@@ -273,7 +278,7 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
     }
     // T(i-1): beta_1 (D, E), candidates P1 (right, a1) and P0 (left, a1)
     {
-        const double Db = 26.0 * D1 * pb1, Eb = 6.0 * E1 * pb1;
+        const double Db = t1 * pb1, Eb = e1 * pb1;
         const double Pr = a1 * sgr, Pl = a1 * sgl;
         acc[1] += fma_(2.0, Pl, Db + Eb - Pr);
         acc[2] += fma_(5.0, Pr + Pl, -2.0 * Db);
@@ -281,13 +286,13 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
     }
     // T(i): beta_0 (D, X = 2D + E), candidates P0 (right, a0) and Pm (left, m0)
     {
-        const double Db = 26.0 * D0 * pb0, Xb = 6.0 * X0 * pb0;
+        const double Db = t0 * pb0, Xb = x0 * pb0;
         const double Pr = a0 * sgr, Pl = m0 * sgl;
         acc[2] += fma_(11.0, Pl, fma_(2.0, Pr, fma_(3.0, Xb, Db)));
         acc[3] += fma_(-7.0, Pl, fma_(5.0, Pr, fma_(-4.0, Xb, -2.0 * Db)));
         acc[4] += fma_(2.0, Pl, Db + Xb - Pr);
     }
-    return 0.5 * fma_(Gp, vr, -(Gm * vl));
+    return fma_(Gp, vr, -(Gm * vl));
 }
 
 // K = 2 from the pairs (v_{i-1}, v_i) and (v_i, v_{i+1}); acc[0..2] = cells i-1..i+1.
@@ -305,8 +310,8 @@ __device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pa
     vr = nr * ir;
     vl = nl * il;
     const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
-    const double gr = 0.5 * Gp * (vr + alpha), gl = 0.5 * Gm * (vl - alpha);
-    abar = 0.5 * fma_(Gp, vr, -(Gm * vl));
+    const double gr = Gp * (vr + alpha), gl = Gm * (vl - alpha);
+    abar = fma_(Gp, vr, -(Gm * vl));
     const double sgr = gr * ir, sgl = gl * il;
     const double vr2 = 2.0 * vr, vl2 = 2.0 * vl;
     const double iq = rcp_nr(q0 * q1);
@@ -444,13 +449,13 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
             if (cell[i] == q0.j) mu[i] += d;
         if (edge && hc == q0.j) lam_edge += d;
     }
-    // phi = lam / width for the lane's cells, neighbours' from the adjacent lanes; on a grid
-    // whose edges are all exact every width is rx, and 1/rx is formed once (the same value)
-    const double iw_const = constw ? rcp_nr(rx) : 0.0;
+    // phi = lam / (2 width) for the lane's cells, neighbours' from the adjacent lanes; on a grid
+    // whose edges are all exact every width is rx, and 1/(2rx) is formed once (the same value)
+    const double iw_const = constw ? 0.5 * rcp_nr(rx) : 0.0;
     auto inv_w = [&](long long g) {
         if (constw) return iw_const;
         const double w = linear_edge(xmin, rx, g + 1) - linear_edge(xmin, rx, g);   // src/hrweno_grids.f90:76-78,247
-        return rcp_nr(w);
+        return 0.5 * rcp_nr(w);
     };
 #pragma unroll
     for (int i = 0; i < R; ++i) phi[i + 1] = mu[i] * inv_w(cell[i]);
@@ -495,9 +500,9 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
         for (int t = 0; t < R; ++t) {
             const double vv = v[t];
             const double Gp = phi[t + 2] - phi[t + 1], Gm = phi[t + 1] - phi[t];
-            acc[t] += 0.5 * Gp * (vv + alpha) + 0.5 * Gm * (vv - alpha);
+            acc[t] += Gp * (vv + alpha) + Gm * (vv - alpha);
             if (owned && c0 + t < n) {
-                abar += 0.5 * (Gp - Gm) * vv;
+                abar += (Gp - Gm) * vv;
                 if (ALPHAMAX && fabs(vv) == alpha && cnd == 0xffffffffu) cnd = (unsigned)(c0 + t);
             }
         }
