@@ -409,7 +409,7 @@ def run_adjoint_general(args, torch, hh, W, dev):
         f"{ms_b:.2f} ms, plain forward {ms_p:.2f} ms; mstvd recording {ms_mf:.2f} ms, adjoint "
         f"{ms_mb:.2f} ms, plain forward {ms_mp:.2f} ms")
     rl = roofline(88.0 / 3 * n, ms_b / (3 * steps),
-                  "hrw::rhs_vjp_kernel<3,GODUNOV,zero-gradient,fixed>", "adjoint_general_2p20")
+                  "hrw::rhs_vjp5_kernel<GODUNOV,zero-gradient>", "adjoint_general_2p20")
     config = {"workload": "burgers1d_example1_godunov_zerograd_weno5_adjoint_2^20cells_400steps",
               "cells": n, "steps": steps, "k": 3, "eps": 1e-6,
               "loss": "0.5*|u_N - F(u0 + 0.01 sin(3 pi x))|^2", "parallelism": "single GPU",

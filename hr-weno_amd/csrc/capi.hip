@@ -1006,11 +1006,54 @@ struct GenTerm {
     double e;
 };
 // out = sum e_i x_i + ch J(s)^T (b1 y1 + b2 y2) for problem p (the Lax-Friedrichs alpha of s
-// taken as max|s| when the problem says so); out must not alias y1 or y2.
+// taken as max|s| when the problem says so); out must not alias y1 or y2.  K = 3 without a
+// line-wide alpha takes the register-blocked pass (rhs_vjp5_kernel) unless the problem is tuned
+// to the LDS-tile kernels (variant 1); everything else the LDS-tile pass (rhs_vjp_kernel).
 static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const double *s,
                    const double *y1, double b1, const double *y2, double b2,
                    std::initializer_list<GenTerm> terms, double *out, double ch)
 {
+    const bool amax = p->alphamax();
+    if (p->k == 3 && !amax && p->variant != 1 && p->neq > 0) {
+        // the register-blocked pass (pairs of cells as 16-byte loads and stores): the terms on
+        // y1 or y2 become their coefficients
+        BvjpArgs b{};
+        b.s = s;
+        b.y1 = y1;
+        b.b1 = b1;
+        b.y2 = y2;
+        b.b2 = y2 ? b2 : 0.0;
+        int nx = 0;
+        bool fits = true;
+        for (const GenTerm &q : terms) {
+            if (!q.x) continue;
+            if (q.x == y1) b.ey1 += q.e;
+            else if (y2 && q.x == y2) b.ey2 += q.e;
+            else if (nx < 2) {
+                b.x[nx] = q.x;
+                b.ex[nx] = q.e;
+                ++nx;
+            } else {
+                fits = false;
+            }
+        }
+        fits = fits && aligned16(s) && aligned16(y1) && aligned16(y2) && aligned16(b.x[0]) &&
+               aligned16(b.x[1]) && aligned16(out);
+        if (fits) {
+            b.out = out;
+            b.ch = ch;
+            b.eps = p->eps;
+            b.alpha = p->alpha;
+            b.xmin = p->xmin;
+            b.rx = p->rx;
+            b.constw = exact_linear_grid(p->xmin, p->rx, p->neq) ? 1 : 0;
+            b.n = p->neq;
+            b.nchunks = (p->neq + 62LL * kBvjpR - 1) / (62LL * kBvjpR);
+            launch_rhs_vjp5(p->numflux, p->bc == HRW_BC_PERIODIC, b, st);
+            LAUNCH_CHECK();
+            return HRW_OK;
+        }
+    }
     GenAdjArgs a{};
     a.s = s;
     a.y1 = y1;
@@ -1031,7 +1074,6 @@ static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const 
     a.rx = p->rx;
     a.eps = p->eps;
     a.alpha = p->alpha;
-    const bool amax = p->alphamax();
     if (amax) {
         clear_slots_kernel<<<1, 256, 0, st>>>(g.aset, 1);
         maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, st>>>(s, p->neq, g.aset);

@@ -224,13 +224,15 @@ __device__ __forceinline__ PendAt pend_at(unsigned tag, const LineRec &r, int i)
     return PendAt{khi == tag ? (int)(0xffffffffu - klo) : -1, d};
 }
 
-// Reverse of one K = 3 reconstruction of cell i from its window x[0..4] = v_{i-2..i+2}, for
-// the halved flux cotangents phi3 = (phi_{i-1}, phi_i, phi_{i+1}), phi = lam/(2w) (the 1/2 of
-// the Lax-Friedrichs flux folded in: an exact scaling).  The triple pieces are rebuilt from the
-// window for each cell (cheaper in registers than carrying them between cells).  The window
-// cotangent is added to acc[0..4]; returns the cell's share of alpha-bar.
-__device__ __forceinline__ double recon3_vjp(double eps12, const double *x, const double *phi3,
-                                             double alpha, double *acc)
+// Reverse of one K = 3 reconstruction of cell i from its window x[0..4] = v_{i-2..i+2}.  The
+// cotangents of the two faces come from face_cot(vr, vl, ir, il, sgr, sgl), which sets
+// sgr = vr-bar * ir and sgl = vl-bar * il (ir, il: the reciprocals of the two sums of weights,
+// sr = 6 sum_r (d_r Q_r), vr = nr/sr).  The triple pieces are rebuilt from the window for each
+// cell (cheaper in registers than carrying them between cells).  The window cotangent is added
+// to acc[0..4].
+template <class FaceCot>
+__device__ __forceinline__ void weno5_vjp(double eps12, const double *x, FaceCot &&face_cot,
+                                          double *acc)
 {
     // T(i-2) = x0..x2 (beta_2, right cand P2, left cand P1), T(i-1) = x1..x3 (beta_1, P1, P0),
     // T(i) = x2..x4 (beta_0, P0, Pm); with D = a - 2b + c, E = a - c:
@@ -260,8 +262,8 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
     const double ir = rcp_nr(sr), il = rcp_nr(sl);
     const double vr = fma_(a0, cr0, fma_(a1, cr1, m2 * cr2)) * ir;
     const double vl = fma_(m0, cl0, fma_(a1, cl1, l2 * cl2)) * il;
-    const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
-    const double sgr = (Gp * (vr + alpha)) * ir, sgl = (Gm * (vl - alpha)) * il;
+    double sgr, sgl;
+    face_cot(vr, vl, ir, il, sgr, sgl);
     const double vr6 = 6.0 * vr, vl6 = 6.0 * vl;
     // twice the cotangent of p_r = 12 (eps + beta_r):  -4/p_r [ w-numerator_r sgr (cr_r - 6vr) + ... ]
     const double ipi = -4.0 * rcp_nr(p0 * p1 * p2);
@@ -292,7 +294,24 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
         acc[3] += fma_(-7.0, Pl, fma_(5.0, Pr, fma_(-4.0, Xb, -2.0 * Db)));
         acc[4] += fma_(2.0, Pl, Db + Xb - Pr);
     }
-    return fma_(Gp, vr, -(Gm * vl));
+}
+
+// The Lax-Friedrichs form, for the halved flux cotangents phi3 = (phi_{i-1}, phi_i, phi_{i+1}),
+// phi = lam/(2w) (the 1/2 of the flux folded in: an exact scaling): vr-bar = (phi_{i+1} - phi_i)
+// (vr + alpha), vl-bar = (phi_i - phi_{i-1}) (vl - alpha).  Returns the cell's share of alpha-bar.
+__device__ __forceinline__ double recon3_vjp(double eps12, const double *x, const double *phi3,
+                                             double alpha, double *acc)
+{
+    double abar;
+    weno5_vjp(eps12, x,
+              [&](double vr, double vl, double ir, double il, double &sgr, double &sgl) {
+                  const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
+                  sgr = (Gp * (vr + alpha)) * ir;
+                  sgl = (Gm * (vl - alpha)) * il;
+                  abar = fma_(Gp, vr, -(Gm * vl));
+              },
+              acc);
+    return abar;
 }
 
 // K = 2 from the pairs (v_{i-1}, v_i) and (v_i, v_{i+1}); acc[0..2] = cells i-1..i+1.

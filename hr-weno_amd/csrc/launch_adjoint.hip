@@ -44,6 +44,19 @@ void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjA
     else lgen_k<3>(flux, periodic, alphamax, g, s);
 }
 
+void launch_rhs_vjp5(int flux, bool periodic, const BvjpArgs &g, hipStream_t s)
+{
+    if (g.nchunks <= 0) return;
+    const unsigned grid = (unsigned)((g.nchunks + 3) / 4);
+    if (flux == LF) {
+        if (periodic) rhs_vjp5_kernel<LF, true><<<grid, 256, 0, s>>>(g);
+        else rhs_vjp5_kernel<LF, false><<<grid, 256, 0, s>>>(g);
+    } else {
+        if (periodic) rhs_vjp5_kernel<GODUNOV, true><<<grid, 256, 0, s>>>(g);
+        else rhs_vjp5_kernel<GODUNOV, false><<<grid, 256, 0, s>>>(g);
+    }
+}
+
 void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s)
 {
     const int nblocks = (int)((g.n + kGenTile - 1) / kGenTile);

@@ -272,8 +272,8 @@ class Problem:
         return n.value
 
     def tune(self, variant: int = 0, cells_per_lane: int = 0):
-        """Kernel choice of the 1D FAST form (0: register-blocked, 1: LDS tile, 2: persistent;
-        0 with cells_per_lane 0: chosen by the line length);
+        """Kernel choice of the 1D FAST form (0: register-blocked, 1: LDS tile, also for the
+        adjoint's reverse passes, 2: persistent; 0 with cells_per_lane 0: chosen by the line length);
         for a 2D problem, variant 0 and cells_per_lane = rows per wavefront (0: automatic)."""
         call("hrw_problem_tune", self._h, int(variant), int(cells_per_lane))
         return self

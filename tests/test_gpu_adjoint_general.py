@@ -1,5 +1,6 @@
 """Device discrete adjoint beyond periodic Lax-Friedrichs lines, through the C-ABI: the general
-reverse pass (rhs_vjp_kernel) for the Godunov flux and zero-gradient ends under rktvd
+reverse pass (rhs_vjp_kernel, and rhs_vjp5_kernel for K = 3 without a line-wide alpha) for the
+Godunov flux and zero-gradient ends under rktvd
 (hrw_adjoint_*), and the recorded multistep integrator (hrw_ode_tape / hrw_ode_adjoint on an
 mstvd object) for every composition -- against the oracle adjoint, the closed forms the oracle
 is pinned by, and central differences of the device forward."""
@@ -135,6 +136,41 @@ def test_conserved_total_and_fixed_ends_on_device(hh, dev, k, flux):
         assert uN[end] == u0[end] and not g[rest].any() and abs(g[end] - 1.0) <= 4e-16
         uN, g, _ = _ms_device(hh, dev, u0, e, 20.5 * dt, dt, k=k, flux=flux, bc=ZG)
         assert uN[end] == u0[end] and not g[rest].any() and abs(g[end] - 1.0) <= 4e-16
+
+
+@pytest.mark.parametrize("flux,bc,alpha", [(GOD, PER, -1.0), (GOD, ZG, -1.0), (LF, PER, 1.7),
+                                           (LF, ZG, 1.7)])
+def test_blocked_reverse_matches_tile_reverse(hh, dev, flux, bc, alpha):
+    """K = 3 without a line-wide alpha takes the register-blocked reverse pass (124 cells per
+    wavefront); variant 1 of the problem keeps the LDS-tile pass.  With the EXACT forward (the
+    same states under both) the gradients agree to rounding: rktvd (one cotangent and pointwise
+    terms; periodic Lax-Friedrichs takes its own kernel there) and the multistep integrator
+    (two cotangents), on lengths shorter than the stencil, around one and two chunks, around the
+    first and last interior chunk, on grids with exact (2^m cells) and inexact edges.  The two
+    passes round differently (reciprocals against divides, products of the weights against
+    normalised weights), which the derivatives through the nonlinear weights amplify over the
+    42 reverse stages to ~1e-12: held to TOL, as against the oracle."""
+    for n in (1, 2, 4, 5, 123, 124, 125, 130, 248, 249, 251, 375, 1000, 1024, 20000, 1 << 20,
+              (1 << 20) + 3):
+        u0, w, dt = _case(n, 500 + n)
+        gN = np.random.default_rng(n + 7).standard_normal(n)
+        got = {}
+        for variant in (0, 1):
+            def prob():
+                return hh.Burgers1D(n, k=3, flux=flux, bc=bc, alpha=alpha,
+                                    arith="exact").tune(variant)
+            r = []
+            if not (flux == LF and bc == PER):
+                adj = hh.Adjoint(prob(), nsteps=3, dt=dt, order=3)
+                adj.forward(_dev(u0, dev))
+                r.append(adj.backward(_dev(gN, dev)).cpu().numpy())
+            ode = hh.MsTvd(prob()).tape(1)
+            u = _dev(u0, dev)
+            ode.integrate(u, 0.0, 13.5 * dt, dt)
+            r.append(ode.adjoint(_dev(gN, dev)).cpu().numpy())
+            got[variant] = r
+        for g0, g1 in zip(got[0], got[1]):
+            assert _close(g0, g1), (n, np.max(np.abs(g0 - g1)) / np.abs(g1).max())
 
 
 def _richardson_fd(fwd, u0, target, j, h=1e-6):
