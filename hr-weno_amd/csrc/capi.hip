@@ -1048,7 +1048,7 @@ static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const 
             b.rx = p->rx;
             b.constw = exact_linear_grid(p->xmin, p->rx, p->neq) ? 1 : 0;
             b.n = p->neq;
-            b.nchunks = (p->neq + 62LL * kBvjpR - 1) / (62LL * kBvjpR);
+            b.nchunks = (p->neq + 60LL * kBvjpR - 1) / (60LL * kBvjpR);
             launch_rhs_vjp5(p->numflux, p->bc == HRW_BC_PERIODIC, b, st);
             LAUNCH_CHECK();
             return HRW_OK;

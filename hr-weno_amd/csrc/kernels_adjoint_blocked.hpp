@@ -6,17 +6,17 @@
 //     out = ey1 y1 + ey2 y2 + ex0 x0 + ex1 x1 + ch J(s)^T (b1 y1 + b2 y2),
 // with the pointwise terms on y1 and y2 folded into their coefficients by the launcher.
 //
-// Layout as the periodic Lax-Friedrichs reverse kernel: wavefront c takes the output cells
-// [W, W + 62R), W = 62Rc; lane l holds the R cells from c0 = W + (l-1)R and their window
-// halo from its neighbours (DPP wave shifts); lanes 0 and 63 are halo lanes that load the
-// cells beyond the wavefront and reconstruct the one extra cell whose face the wavefront's
-// outer interfaces need.  Each lane forms the faces of its cells, the interface cotangents
+// Layout: wavefront c takes the output cells [W, W + 60R), W = 60Rc; lane l holds the R cells
+// from c0 = W + (l-2)R and their window halo from its neighbours (DPP wave shifts).  Lanes 0,
+// 1, 62 and 63 are halo lanes: 1 and 62 run the reverse of the cells whose windows spill into
+// the wavefront's first and last cells, 0 and 63 only supply the faces and phi those need (and
+// load the H cells beyond the wavefront), so no lane reconstructs a cell twice.  Each lane
+// forms the faces of its cells, the interface cotangents
 // (phi = (b1 y1 + b2 y2)/w, the cotangent of F_{p+1/2} is phi_{p+1} - phi_p):
 //     vr-bar_p = (phi_{p+1} - phi_p) dF/dvm (vr_p, vl_{p+1}),
 //     vl-bar_p = (phi_p - phi_{p-1}) dF/dvp (vr_{p-1}, vl_p),
 // runs the reverse of each reconstruction into R + 4 accumulators and hands the two that spill
-// past each end to the neighbouring lane.  No LDS, no barrier: 11-12 us per 2^20 cells
-// against 18-23 us for the LDS-tile pass.
+// past each end to the neighbouring lane.  No LDS, no barrier.
 // Ends: with periodic ends every position stands for its periodic image (every (cell, offset)
 // pair of the stencil is counted once, also for lines shorter than the stencil).  With
 // zero-gradient ends the end cells' RHS is zero, so phi vanishes at cells 0 and n-1 and past
@@ -54,12 +54,10 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
 {
     constexpr int R = kBvjpR, H = 2, NV = R + 2 * H;
     const long long n = a.n;
-    const long long c0 = W + (long long)(lane - 1) * R;
-    const bool lo = lane < 32;
-    // the edge lanes' cells beyond the wavefront: hv[0..H] = the H+1 cells left of lane 0's
-    // window (lane 0) or right of lane 63's cells (lane 63); ce = the cell just outside
-    const long long hc = lo ? W - R - H - 1 : W + 63LL * R;
-    const long long ce = lo ? W - R - 1 : W + 63LL * R;
+    const long long c0 = W + (long long)(lane - 2) * R;
+    // the edge lanes' window halo beyond the wavefront: the H cells left of lane 0's cells
+    // (lanes < 32 load them) or right of lane 63's
+    const long long hc = lane < 32 ? W - 2 * R - H : W + 62LL * R;
     auto at = [&](long long c) { return INTERIOR ? c : cell_index(c, n, PERIODIC); };
     // 1/w; zero-gradient ends: phi is zero at the end cells and past them
     const double iwc = a.constw ? rcp_nr(a.rx) : 0.0;
@@ -70,7 +68,7 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
         return lj * rcp_nr(linear_edge(a.xmin, a.rx, g + 1) - linear_edge(a.xmin, a.rx, g));
     };
 
-    double v[NV], y1[R], y2[R], hv[H + 1];
+    double v[NV], y1[R], y2[R], hv[H];
     double o[R];   // the pointwise terms
     if constexpr (INTERIOR) {
         load_run<R>(a.s + c0, v + H);
@@ -79,14 +77,13 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
         for (int i = 0; i < R; ++i) v[H + i] = a.s[at(c0 + i)];
     }
 #pragma unroll
-    for (int i = 0; i < H + 1; ++i) hv[i] = a.s[at(hc + i)];
+    for (int i = 0; i < H; ++i) hv[i] = a.s[at(hc + i)];
     if constexpr (INTERIOR) {
         load_run<R>(a.y1 + c0, y1);
     } else {
 #pragma unroll
         for (int i = 0; i < R; ++i) y1[i] = a.y1[at(c0 + i)];
     }
-    double lje = a.b1 * a.y1[at(ce)];
     if (a.y2) {
         if constexpr (INTERIOR) {
             load_run<R>(a.y2 + c0, y2);
@@ -94,7 +91,6 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
 #pragma unroll
             for (int i = 0; i < R; ++i) y2[i] = a.y2[at(c0 + i)];
         }
-        lje = fma_(a.b2, a.y2[at(ce)], lje);
     } else {
 #pragma unroll
         for (int i = 0; i < R; ++i) y2[i] = 0.0;
@@ -123,41 +119,25 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
     }
     if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < H; ++i) v[i] = hv[i + 1];
+        for (int i = 0; i < H; ++i) v[i] = hv[i];
     }
     if (lane == 63) {
 #pragma unroll
         for (int i = 0; i < H; ++i) v[H + R + i] = hv[i];
     }
 
-    // faces of the lane's cells (triples shared between neighbouring cells); the edge lanes
-    // also reconstruct the cell next to the wavefront: c0-1 (lane 0) or c0+R (lane 63)
+    // faces of the lane's cells (triples shared between neighbouring cells) and the
+    // neighbours' (lanes 0 and 63 get their own back: unused)
     const double eps12 = 12.0 * a.eps;
-    double vr[R], vl[R], vr_prev, vl_next;
+    double vr[R], vl[R];
     {
         Tri3 T[R + 2];
 #pragma unroll
         for (int j = 0; j < R + 2; ++j) T[j] = tri3(v[j], v[j + 1], v[j + 2]);
 #pragma unroll
         for (int t = 0; t < R; ++t) recon3(eps12, T[t], T[t + 1], T[t + 2], vl[t], vr[t]);
-        vr_prev = shfl_up1(vr[R - 1]);
-        vl_next = shfl_down1(vl[0]);
-        double xw[5];
-        if (lo) {
-            xw[0] = hv[0];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) xw[i + 1] = v[i];
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) xw[i] = v[R + i];
-            xw[4] = hv[H];
-        }
-        double el, er;
-        recon3(eps12, tri3(xw[0], xw[1], xw[2]), tri3(xw[1], xw[2], xw[3]),
-               tri3(xw[2], xw[3], xw[4]), el, er);
-        if (lane == 0) vr_prev = er;
-        if (lane == 63) vl_next = el;
     }
+    const double vr_prev = shfl_up1(vr[R - 1]), vl_next = shfl_down1(vl[0]);
     // flux derivatives at faces f = 0..R (between local cells f-1 and f)
     double dm[R + 1], dp[R + 1];
 #pragma unroll
@@ -172,11 +152,6 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
     for (int i = 0; i < R; ++i) phi[i + 1] = phi_of(c0 + i, fma_(a.b2, y2[i], a.b1 * y1[i]));
     phi[0] = shfl_up1(phi[R]);
     phi[R + 1] = shfl_down1(phi[1]);
-    {
-        const double pe = phi_of(ce, lje);
-        if (lane == 0) phi[0] = pe;
-        if (lane == 63) phi[R + 1] = pe;
-    }
 
     double acc[NV];
 #pragma unroll
@@ -203,7 +178,8 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
     if constexpr (!INTERIOR && !PERIODIC) {
         // window positions past an end are the end cell: positions c0-2, c0-1 of the lane
         // holding cell 0 (only its own cells' windows reach them) and positions n, n+1 of the
-        // lane holding cell n-1 (its own cells' and, through the exchange, its left neighbour's)
+        // lane holding cell n-1 (its own cells' and, through the exchange, its left neighbour's;
+        // the cells right of n-1 are images with phi = 0 on both sides, which add nothing)
         if (c0 == 0) acc[H] += acc[0] + acc[1];
         const long long jl = n - 1 - c0;
         if (jl >= 0 && jl < R) {
@@ -216,7 +192,7 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
                 if (i == jl) acc[H + i] += t;
         }
     }
-    if (lane >= 1 && lane <= 62) {
+    if (lane >= 2 && lane <= 61) {
         double r[R];
 #pragma unroll
         for (int i = 0; i < R; ++i) r[i] = fma_(a.ch, acc[H + i], o[i]);
@@ -230,9 +206,8 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
     }
 }
 
-// One wavefront per chunk of 62R output cells.  3 waves per SIMD (up to 168 VGPRs): the reverse
-// of a reconstruction holds ~60 values at its peak on top of the lane's window, faces and
-// accumulators.
+// One wavefront per chunk of 60R output cells.  At least 3 waves per SIMD (up to 168 VGPRs);
+// the kernel takes 118-120, so 4 fit.  Asking for 5 (96 VGPRs) spills 33-48 registers.
 template <int FLUX, bool PERIODIC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void rhs_vjp5_kernel(
     BvjpArgs a)
@@ -241,9 +216,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
     const long long c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (c >= a.nchunks) return;
     const int lane = threadIdx.x & 63;
-    const long long W = c * 62LL * R;
-    // every cell the wavefront touches lies in [W - R - H - 1, W + 63R + H]
-    if (W - R - H - 1 >= 0 && W + 63LL * R + H <= a.n - 1)
+    const long long W = c * 60LL * R;
+    // every cell the wavefront touches lies in [W - 2R - H, W + 62R + H - 1]
+    if (W - 2 * R - H >= 0 && W + 62LL * R + H - 1 <= a.n - 1)
         bvjp_chunk<FLUX, PERIODIC, true>(a, W, lane);
     else
         bvjp_chunk<FLUX, PERIODIC, false>(a, W, lane);

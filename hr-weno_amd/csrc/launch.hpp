@@ -52,7 +52,7 @@ void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjA
                     hipStream_t s);
 void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s);
 // The same for K = 3 without a line-wide alpha (Godunov, or Lax-Friedrichs with a fixed alpha),
-// register-blocked: one wavefront per 62*kBvjpR cells.
+// register-blocked: one wavefront per 60*kBvjpR cells.
 void launch_rhs_vjp5(int flux, bool periodic, const BvjpArgs &g, hipStream_t s);
 
 }  // namespace hrw
