@@ -365,6 +365,7 @@ def run_adjoint_general(args, torch, hh, W, dev):
     c, _ = W.config5(steps=args.steps)
     n, steps, dt = c.n, c.steps, c.dt
     prob = hh.Burgers1D(n, flux=hh.GODUNOV, bc=hh.BC_ZERO_GRADIENT)
+    prob.tune(args.variant, args.cells_per_lane)     # variant 1: the LDS-tile kernels both ways
     adj = hh.Adjoint(prob, nsteps=steps, dt=dt, ckpt_every=steps)
     u0 = torch.as_tensor(c.u0, device=dev).clone()
     # the target from a smooth perturbation: with per-cell noise the endpoint Godunov flux
@@ -408,8 +409,11 @@ def run_adjoint_general(args, torch, hh, W, dev):
     log(f"adjoint (example 1 composition): rktvd forward(recording) {ms_f:.2f} ms, backward "
         f"{ms_b:.2f} ms, plain forward {ms_p:.2f} ms; mstvd recording {ms_mf:.2f} ms, adjoint "
         f"{ms_mb:.2f} ms, plain forward {ms_mp:.2f} ms")
+    tile = args.variant == 1
     rl = roofline(88.0 / 3 * n, ms_b / (3 * steps),
-                  "hrw::rhs_vjp5_kernel<GODUNOV,zero-gradient>", "adjoint_general_2p20")
+                  "hrw::rhs_vjp_kernel<3,GODUNOV,zero-gradient,fixed>" if tile
+                  else "hrw::rhs_vjp5_kernel<GODUNOV,zero-gradient>",
+                  "adjoint_general_2p20_tile" if tile else "adjoint_general_2p20")
     config = {"workload": "burgers1d_example1_godunov_zerograd_weno5_adjoint_2^20cells_400steps",
               "cells": n, "steps": steps, "k": 3, "eps": 1e-6,
               "loss": "0.5*|u_N - F(u0 + 0.01 sin(3 pi x))|^2", "parallelism": "single GPU",
@@ -483,7 +487,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0,
-                    help="1D FAST kernel: 0 register-blocked, 1 LDS tile")
+                    help="1D FAST kernel: 0 register-blocked, 1 LDS tile (also the "
+                         "adjoint_general leg's reverse pass)")
     ap.add_argument("--cells-per-lane", type=int, default=0,
                     help="cells per lane of the 1D stage kernel (0: chosen by the line length)")
     ap.add_argument("--per-line-steps", type=int, default=20)
