@@ -70,10 +70,14 @@ __device__ __forceinline__ Tri3 tri3(double a, double b, double c)
 
 // One reconstruction from the pieces of T(j-2), T(j-1), T(j).  eps12 = 12*eps.  Each face
 // takes one reciprocal of its own sum of weights, so the only products are pairs of q's: finite
-// while beta < ~1e77, i.e. for cell-value jumps below ~1e38.  SCALED: vl and vr come out
+// while beta < ~1e77, i.e. for cell-value jumps below ~1e38.  SHARED: the two faces share one
+// reciprocal, of the product of their sums of weights (within a factor 3 of each other): one
+// transcendental and one FP64 operation fewer per cell, two more values live across it.  With
+// eps above machine epsilon every q is at least (12 eps)^2 ~ 1e-29, so the product stays above
+// ~1e-115; it stays finite while beta < ~1e37 (cell-value jumps below ~1e18).  SCALED: vl and vr come out
 // multiplied by 6 (the candidates are 6 c(:,r).T; the caller's flux constants absorb the
 // factor, two multiplies fewer per cell).
-template <bool SCALED = false>
+template <bool SCALED = false, bool SHARED = false>
 __device__ __forceinline__ void recon3(double eps12, const Tri3 &t2, const Tri3 &t1,
                                        const Tri3 &t0, double &vl, double &vr)
 {
@@ -91,8 +95,14 @@ __device__ __forceinline__ void recon3(double eps12, const Tri3 &t2, const Tri3 
     }
     const double nr = fma_(a0, t0.P0, fma_(a1, t1.P1, m2 * t2.P2));
     const double nl = fma_(m0, t0.Pm, fma_(a1, t1.P0, l2 * t2.P1));
-    vr = nr * rcp_nr(sr);
-    vl = nl * rcp_nr(sl);
+    if constexpr (SHARED) {
+        const double i2 = rcp_nr(sr * sl);
+        vr = (nr * sl) * i2;
+        vl = (nl * sr) * i2;
+    } else {
+        vr = nr * rcp_nr(sr);
+        vl = nl * rcp_nr(sl);
+    }
 }
 
 // K = 2: pair (a, b) = (v_m, v_m+1) gives beta_0(m) = beta_1(m+1) = (b-a)^2 and the
@@ -401,7 +411,7 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         for (int t = 2; t < R; ++t) {
             const Tri3 t0 = tri3(v[t + 2], v[t + 3], v[t + 4]);   // T(c0+t)
             double vl, vr;
-            recon3<true>(eps12, t2, t1, t0, vl, vr);
+            recon3<true, true>(eps12, t2, t1, t0, vl, vr);
             t2 = t1;
             t1 = t0;
             if (t == 2) {
@@ -446,8 +456,8 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         l1.P2 = shfl_up1(t1.P2);
         const double vr_left = shfl_up1(vr_prev);   // vr of cell c0-1 (lane-1's last)
         double vl0, vr0, vl1, vr1;
-        recon3<true>(eps12, l2, l1, ta, vl0, vr0);
-        recon3<true>(eps12, l1, ta, tb, vl1, vr1);
+        recon3<true, true>(eps12, l2, l1, ta, vl0, vr0);
+        recon3<true, true>(eps12, l1, ta, tb, vl1, vr1);
         const double vl_right = shfl_down1(vl0);    // vl of cell c0+R (lane+1's first)
         const double f0 = numflux(vr_left, vl0);    // face c0-1/2
         const double f1 = numflux(vr0, vl1);        // face c0+1/2
@@ -490,7 +500,7 @@ __device__ __forceinline__ double chunk_stage(const Stage1DArgs &a, int W, doubl
         double vl, vr;
         if constexpr (K == 3) {
             const Tri3 t0 = tri3(v[t + 2], v[t + 3], v[t + 4]);   // T(c0+t)
-            recon3<true>(eps12, t2, t1, t0, vl, vr);
+            recon3<true, true>(eps12, t2, t1, t0, vl, vr);
             t2 = t1;
             t1 = t0;
         } else if constexpr (K == 2) {
@@ -572,12 +582,16 @@ __host__ __device__ constexpr int stage_npw(int stage)
     return stage == MS3 ? 2 : ((stage == RK2_2 || stage == RK3_2 || stage == RK3_3) ? 1 : 0);
 }
 
-// Waves per SIMD each stage kernel is compiled for: the most whose register budget (512 /
-// waves) holds the march without spilling (R = 8 needs 136..168 VGPRs, R = 6 112..144, R = 4
-// 102..120 for 0..2 pointwise operands).
-__host__ __device__ constexpr int fast_waves(int R, int NPW)
+// Waves per SIMD each stage kernel is compiled for (a budget of 512/waves VGPRs per lane).  With
+// K = 3 the kernels take 102..128 VGPRs at R = 4 (0..2 pointwise operands), 120..168 at R = 6
+// and 126..168 at R = 8.  R = 6 runs at 4 waves without a pointwise operand (the
+// Lax-Friedrichs kernel with alpha = max|u| on an exact grid fits in 128 VGPRs; other variants
+// spill) and at 3 with one or two.  The Godunov one-chunk kernel at R = 6 runs at 3 in every
+// stage: at 4, its stage without a pointwise operand spilled 36-48 registers and was 7 % slower.
+__host__ __device__ constexpr int fast_waves(int R, int NPW, int K = 3, int FLUX = LF)
 {
-    return R >= 8 ? 3 : (R == 6 ? (NPW <= 1 ? 4 : 3) : 4);
+    if (K == 3 && FLUX == GODUNOV && R == 6) return 3;
+    return R >= 8 ? 3 : (R == 6 ? (NPW == 0 ? 4 : 3) : 4);
 }
 
 // The start of every stage wave: the alpha partial maxima of the previous launch are loaded
@@ -684,7 +698,7 @@ __device__ __forceinline__ void chunk_wave(const Stage1DArgs &a, double *s_red)
 
 // NPW = stage_npw(a.stage), chosen by the host.
 template <int K, int FLUX, bool PERIODIC, bool ALPHAMAX, int R, int NPW, bool CONSTW>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW))))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fast_waves(R, NPW, K, FLUX))))
 void stage1d_fast_kernel(Stage1DArgs a)
 {
     __shared__ double s_red[kBlock / 64 + 1];

@@ -187,7 +187,11 @@ def test_example1_composition_fd(hh, dev, method):
     """Example 1's own composition (Godunov, zero-gradient ends, WENO5) over 150 steps on 2000
     cells: the device gradient against Richardson central differences of the device forward,
     1e-6 relative per cell (components below 1 % of the largest held to 1e-8 of it: round-off
-    of the differenced trajectories and the endpoint flux's kinks, src/hrweno_fluxes.f90:66-74)."""
+    of the differenced trajectories and the endpoint flux's kinks, src/hrweno_fluxes.f90:66-74).
+    The multistep run amplifies perturbations about 1e9-fold over these steps (its gradient
+    reaches ~1e9, and a change in the forward's last bits moves it several-fold), so its
+    differenced trajectories carry ~steps * eps * max|g| / h, 1.5e-8 of the largest component,
+    as rounding noise: its components below 10 % of the largest are held to 1e-7 of it."""
     n, steps = 2000, 150
     u0, w, dt = _case(n, 51)
     rng = np.random.default_rng(52)
@@ -209,7 +213,7 @@ def test_example1_composition_fd(hh, dev, method):
         g = adj.backward(_dev(r, dev)).cpu().numpy()
     else:
         _, g, _ = _ms_device(hh, dev, u0, r, (steps - 0.5) * dt, dt, flux=GOD, bc=ZG)
-    floor = 1e-2 * np.abs(g).max()
+    floor = (1e-2 if method == "rk" else 1e-1) * np.abs(g).max()
     cells = list(rng.choice(n, 24, replace=False)) + [0, 1, n - 2, n - 1]
     for j in cells:
         fd = _richardson_fd(fwd, u0, target, j)
