@@ -259,7 +259,9 @@ __device__ __forceinline__ void weno5_vjp(double eps12, const double *x, FaceCot
     const double m0 = Q1 * Q2, m1 = Q0 * Q2, m2 = Q0 * Q1;
     const double a0 = 3.0 * m0, a1 = 6.0 * m1, l2 = 3.0 * m2;
     const double sr = 6.0 * (a0 + a1 + m2), sl = 6.0 * (m0 + a1 + l2);
-    const double ir = rcp_nr(sr), il = rcp_nr(sl);
+    // one reciprocal for both faces (sr and sl lie within a factor 3 of each other; see recon3)
+    const double i2 = rcp_nr(sr * sl);
+    const double ir = sl * i2, il = sr * i2;
     const double vr = fma_(a0, cr0, fma_(a1, cr1, m2 * cr2)) * ir;
     const double vl = fma_(m0, cl0, fma_(a1, cl1, l2 * cl2)) * il;
     double sgr, sgl;

@@ -135,7 +135,7 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
 #pragma unroll
         for (int j = 0; j < R + 2; ++j) T[j] = tri3(v[j], v[j + 1], v[j + 2]);
 #pragma unroll
-        for (int t = 0; t < R; ++t) recon3(eps12, T[t], T[t + 1], T[t + 2], vl[t], vr[t]);
+        for (int t = 0; t < R; ++t) recon3<false, true>(eps12, T[t], T[t + 1], T[t + 2], vl[t], vr[t]);
     }
     const double vr_prev = shfl_up1(vr[R - 1]), vl_next = shfl_down1(vl[0]);
     // flux derivatives at faces f = 0..R (between local cells f-1 and f)
