@@ -37,7 +37,8 @@ enum hrw_status {
 /* Arithmetic form.  EXACT keeps the reference's evaluation order (bit-identical to the
  * reference Fortran compiled without FMA).  FAST is the same algebra with fused multiply-adds,
  * the nonlinear weights over a common denominator, work shared between neighbouring cells and
- * reciprocals by hardware estimate plus Newton steps: a few ulp per reconstruction.  Measured
+ * reciprocals by hardware estimate plus Newton steps (one shared by the two faces of a 1D
+ * reconstruction): a few ulp per reconstruction.  Measured
  * agreement with the reference arithmetic: within 1e-12 of max|u| per cell (and 1e-12
  * relative where |u| >= 1e-3 max|u|) for RK1-3 at every horizon tested (100 RK3 steps at 2^24
  * cells, ~1250 to t = 0.5 at 1000 cells, 200 on the 4096-line ensemble).  The reference's own
