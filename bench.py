@@ -412,7 +412,7 @@ def run_adjoint_general(args, torch, hh, W, dev):
     tile = args.variant == 1
     rl = roofline(88.0 / 3 * n, ms_b / (3 * steps),
                   "hrw::rhs_vjp_kernel<3,GODUNOV,zero-gradient,fixed>" if tile
-                  else "hrw::rhs_vjp5_kernel<GODUNOV,zero-gradient>",
+                  else "hrw::rhs_vjp5_kernel<GODUNOV,zero-gradient,fixed>",
                   "adjoint_general_2p20_tile" if tile else "adjoint_general_2p20")
     config = {"workload": "burgers1d_example1_godunov_zerograd_weno5_adjoint_2^20cells_400steps",
               "cells": n, "steps": steps, "k": 3, "eps": 1e-6,

@@ -1,6 +1,6 @@
 // kernels_adjoint_blocked.hpp -- register-blocked form of the general reverse pass for the
-// fifth-order reconstruction (K = 3) with a flux that has no line-wide alpha: the endpoint
-// Godunov flux, or Lax-Friedrichs with a fixed alpha; periodic or zero-gradient ends
+// fifth-order reconstruction (K = 3): the endpoint Godunov flux, or Lax-Friedrichs with a fixed
+// alpha or alpha = max|s|; periodic or zero-gradient ends
 // (example/example1_burgers_1d_fv.f90:72-109; src/hrweno_weno.f90:174-216 per cell;
 // src/hrweno_fluxes.f90:42-43, 66-74).  It computes what rhs_vjp_kernel computes,
 //     out = ey1 y1 + ey2 y2 + ex0 x0 + ex1 x1 + ch J(s)^T (b1 y1 + b2 y2),
@@ -24,6 +24,10 @@
 // end cell (src/hrweno_weno.f90:171-173), so the lanes holding cells 0 and n-1 fold the
 // accumulators past the end into them.  Only the first and last wavefronts of a line touch an
 // end; the others take the unchecked path.
+// alpha = max|s| (Lax-Friedrichs): alpha-bar is a sum over the whole line that lands on the
+// first cell attaining the maximum.  Each block writes its share (its waves' sums in a fixed
+// order) and atomicMin-s the first such cell it holds; rhs_vjp_alpha_fixup adds the correction
+// once the pass has finished, as for the LDS-tile pass.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -47,10 +51,16 @@ struct BvjpArgs {
     double xmin, rx;             // linear grid
     int constw;                  // 1: every edge xmin + rx*i is exact, so every width is rx
     long long n, nchunks;
+    const unsigned long long *aset;  // alpha = max|s|: a slot set holding max|s|
+    double *partial;                 // alpha = max|s|: one alpha-bar share per block
+    int *argmax;                     // alpha = max|s|: first cell with |s| == alpha (INT_MAX: none)
 };
 
-template <int FLUX, bool PERIODIC, bool INTERIOR>
-__device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int lane)
+// Returns the lane's share of alpha-bar over its owned cells (AMAX); *cand = the first owned cell
+// with |s| == alpha (0xffffffff: none).
+template <int FLUX, bool PERIODIC, bool INTERIOR, bool AMAX>
+__device__ __forceinline__ double bvjp_chunk(const BvjpArgs &a, long long W, int lane, double alpha,
+                                             unsigned *cand)
 {
     constexpr int R = kBvjpR, H = 2, NV = R + 2 * H;
     const long long n = a.n;
@@ -139,13 +149,11 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
     }
     const double vr_prev = shfl_up1(vr[R - 1]), vl_next = shfl_down1(vl[0]);
     // flux derivatives at faces f = 0..R (between local cells f-1 and f)
-    double dm[R + 1], dp[R + 1];
+    double dm[R + 1], dp[R + 1], da[R + 1];
 #pragma unroll
-    for (int f = 0; f <= R; ++f) {
-        double da;
-        gflux_partials<FLUX>(a.alpha, f == 0 ? vr_prev : vr[f - 1], f == R ? vl_next : vl[f],
-                             dm[f], dp[f], da);
-    }
+    for (int f = 0; f <= R; ++f)
+        gflux_partials<FLUX>(alpha, f == 0 ? vr_prev : vr[f - 1], f == R ? vl_next : vl[f], dm[f],
+                             dp[f], da[f]);
     // phi of cells c0-1 .. c0+R
     double phi[R + 2];
 #pragma unroll
@@ -153,6 +161,18 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
     phi[0] = shfl_up1(phi[R]);
     phi[R + 1] = shfl_down1(phi[1]);
 
+    const bool owned = lane >= 2 && lane <= 61;
+    double abar = 0.0;
+    unsigned cnd = 0xffffffffu;
+    if constexpr (AMAX) {
+        // the right interface of each owned cell (phi vanishes where an interface does not exist)
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+            if (owned && c0 + t < n) {
+                abar = fma_(phi[t + 2] - phi[t + 1], da[t + 1], abar);
+                if (fabs(v[H + t]) == alpha && cnd == 0xffffffffu) cnd = (unsigned)(c0 + t);
+            }
+    }
     double acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = 0.0;
@@ -192,7 +212,7 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
                 if (i == jl) acc[H + i] += t;
         }
     }
-    if (lane >= 2 && lane <= 61) {
+    if (owned) {
         double r[R];
 #pragma unroll
         for (int i = 0; i < R; ++i) r[i] = fma_(a.ch, acc[H + i], o[i]);
@@ -204,24 +224,57 @@ __device__ __forceinline__ void bvjp_chunk(const BvjpArgs &a, long long W, int l
                 if (c0 + i < n) a.out[c0 + i] = r[i];
         }
     }
+    *cand = cnd;
+    return abar;
 }
 
-// One wavefront per chunk of 60R output cells.  At least 3 waves per SIMD (up to 168 VGPRs);
-// the kernel takes 118-120, so 4 fit.  Asking for 5 (96 VGPRs) spills 33-48 registers.
-template <int FLUX, bool PERIODIC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void rhs_vjp5_kernel(
+// One wavefront per chunk of 60R output cells, 4 waves per SIMD (up to 128 VGPRs: 118-128
+// without spills; left at 3, the alpha = max|s| form took 130).  Asking for 5 (96 VGPRs) spills
+// 33-48 registers.
+template <int FLUX, bool PERIODIC, bool AMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void rhs_vjp5_kernel(
     BvjpArgs a)
 {
     constexpr int R = kBvjpR, H = 2;
+    __shared__ double s_ab[4];
+    __shared__ unsigned s_cnd[4];
     const long long c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (c >= a.nchunks) return;
     const int lane = threadIdx.x & 63;
-    const long long W = c * 60LL * R;
-    // every cell the wavefront touches lies in [W - 2R - H, W + 62R + H - 1]
-    if (W - 2 * R - H >= 0 && W + 62LL * R + H - 1 <= a.n - 1)
-        bvjp_chunk<FLUX, PERIODIC, true>(a, W, lane);
-    else
-        bvjp_chunk<FLUX, PERIODIC, false>(a, W, lane);
+    double alpha = a.alpha;
+    if constexpr (AMAX) {
+        unsigned long long m = 0ull;
+#pragma unroll
+        for (int i = 0; i < kNSlot; ++i) {
+            const unsigned long long x = a.aset[i * kSlotStride];
+            m = x > m ? x : m;
+        }
+        alpha = __longlong_as_double((long long)m);
+    }
+    double ab = 0.0;
+    unsigned cnd = 0xffffffffu;
+    if (c < a.nchunks) {   // (whole wavefronts past the last chunk only join the block's sums)
+        const long long W = c * 60LL * R;
+        // every cell the wavefront touches lies in [W - 2R - H, W + 62R + H - 1]
+        if (W - 2 * R - H >= 0 && W + 62LL * R + H - 1 <= a.n - 1)
+            ab = bvjp_chunk<FLUX, PERIODIC, true, AMAX>(a, W, lane, alpha, &cnd);
+        else
+            ab = bvjp_chunk<FLUX, PERIODIC, false, AMAX>(a, W, lane, alpha, &cnd);
+    }
+    if constexpr (AMAX) {
+        // the block's share of alpha-bar in a fixed order, and its first cell at the maximum
+        const double ws = wave_sum(ab);
+        const unsigned wm = wave_umin(cnd);
+        if (lane == 0) {
+            s_ab[threadIdx.x >> 6] = ws;
+            s_cnd[threadIdx.x >> 6] = wm;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            a.partial[blockIdx.x] = ((s_ab[0] + s_ab[1]) + s_ab[2]) + s_ab[3];
+            const unsigned m = min(min(s_cnd[0], s_cnd[1]), min(s_cnd[2], s_cnd[3]));
+            if (m != 0xffffffffu) atomicMin(a.argmax, (int)m);
+        }
+    }
 }
 
 }  // namespace hrw

@@ -44,17 +44,28 @@ void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjA
     else lgen_k<3>(flux, periodic, alphamax, g, s);
 }
 
-void launch_rhs_vjp5(int flux, bool periodic, const BvjpArgs &g, hipStream_t s)
+void launch_rhs_vjp5(int flux, bool periodic, bool alphamax, const BvjpArgs &g, hipStream_t s)
 {
     if (g.nchunks <= 0) return;
     const unsigned grid = (unsigned)((g.nchunks + 3) / 4);
     if (flux == LF) {
-        if (periodic) rhs_vjp5_kernel<LF, true><<<grid, 256, 0, s>>>(g);
-        else rhs_vjp5_kernel<LF, false><<<grid, 256, 0, s>>>(g);
+        if (alphamax) {
+            if (periodic) rhs_vjp5_kernel<LF, true, true><<<grid, 256, 0, s>>>(g);
+            else rhs_vjp5_kernel<LF, false, true><<<grid, 256, 0, s>>>(g);
+        } else {
+            if (periodic) rhs_vjp5_kernel<LF, true, false><<<grid, 256, 0, s>>>(g);
+            else rhs_vjp5_kernel<LF, false, false><<<grid, 256, 0, s>>>(g);
+        }
     } else {
-        if (periodic) rhs_vjp5_kernel<GODUNOV, true><<<grid, 256, 0, s>>>(g);
-        else rhs_vjp5_kernel<GODUNOV, false><<<grid, 256, 0, s>>>(g);
+        if (periodic) rhs_vjp5_kernel<GODUNOV, true, false><<<grid, 256, 0, s>>>(g);
+        else rhs_vjp5_kernel<GODUNOV, false, false><<<grid, 256, 0, s>>>(g);
     }
+}
+
+void launch_rhs_vjp5_fixup(const BvjpArgs &g, hipStream_t s)
+{
+    const int nblocks = (int)((g.nchunks + 3) / 4);
+    rhs_vjp_alpha_fixup<<<1, kGenTile, 0, s>>>(g.partial, nblocks, g.argmax, g.s, g.out, g.ch);
 }
 
 void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s)

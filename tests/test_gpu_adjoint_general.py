@@ -139,10 +139,11 @@ def test_conserved_total_and_fixed_ends_on_device(hh, dev, k, flux):
 
 
 @pytest.mark.parametrize("flux,bc,alpha", [(GOD, PER, -1.0), (GOD, ZG, -1.0), (LF, PER, 1.7),
-                                           (LF, ZG, 1.7)])
+                                           (LF, ZG, 1.7), (LF, PER, -1.0), (LF, ZG, -1.0)])
 def test_blocked_reverse_matches_tile_reverse(hh, dev, flux, bc, alpha):
-    """K = 3 without a line-wide alpha takes the register-blocked reverse pass (124 cells per
-    wavefront); variant 1 of the problem keeps the LDS-tile pass.  With the EXACT forward (the
+    """K = 3 takes the register-blocked reverse pass (120 cells per wavefront; with alpha =
+    max|s| its alpha-bar shares per block and the fixup launch); variant 1 of the problem keeps
+    the LDS-tile pass.  With the EXACT forward (the
     same states under both) the gradients agree to rounding: rktvd (one cotangent and pointwise
     terms; periodic Lax-Friedrichs takes its own kernel there) and the multistep integrator
     (two cotangents), on lengths shorter than the stencil, around one and two chunks, around the
