@@ -16,7 +16,8 @@
 //     vr-bar_p = (phi_{p+1} - phi_p) dF/dvm (vr_p, vl_{p+1}),
 //     vl-bar_p = (phi_p - phi_{p-1}) dF/dvp (vr_{p-1}, vl_p),
 // runs the reverse of each reconstruction into R + 4 accumulators and hands the two that spill
-// past each end to the neighbouring lane.  No LDS, no barrier.
+// past each end to the neighbouring lane.  No LDS tile and no barrier (the alpha = max|s| form
+// has one, for its block sum at the end).
 // Ends: with periodic ends every position stands for its periodic image (every (cell, offset)
 // pair of the stencil is counted once, also for lines shorter than the stencil).  With
 // zero-gradient ends the end cells' RHS is zero, so phi vanishes at cells 0 and n-1 and past
