@@ -1006,15 +1006,15 @@ struct GenTerm {
     double e;
 };
 // out = sum e_i x_i + ch J(s)^T (b1 y1 + b2 y2) for problem p (the Lax-Friedrichs alpha of s
-// taken as max|s| when the problem says so); out must not alias y1 or y2.  K = 3 takes the
+// taken as max|s| when the problem says so); out must not alias y1 or y2.  K = 2, 3 take the
 // register-blocked pass (rhs_vjp5_kernel) unless the problem is tuned to the LDS-tile kernels
-// (variant 1); K = 1, 2 the LDS-tile pass (rhs_vjp_kernel).
+// (variant 1); K = 1 the LDS-tile pass (rhs_vjp_kernel).
 static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const double *s,
                    const double *y1, double b1, const double *y2, double b2,
                    std::initializer_list<GenTerm> terms, double *out, double ch)
 {
     const bool amax = p->alphamax();
-    if (p->k == 3 && p->variant != 1 && p->neq > 0) {
+    if ((p->k == 2 || p->k == 3) && p->variant != 1 && p->neq > 0) {
         // the register-blocked pass (pairs of cells as 16-byte loads and stores): the terms on
         // y1 or y2 become their coefficients
         BvjpArgs b{};
@@ -1056,7 +1056,7 @@ static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const 
                 b.partial = g.partial;
                 b.argmax = g.argmax;
             }
-            launch_rhs_vjp5(p->numflux, p->bc == HRW_BC_PERIODIC, amax, b, st);
+            launch_rhs_vjp5(p->k, p->numflux, p->bc == HRW_BC_PERIODIC, amax, b, st);
             if (amax) launch_rhs_vjp5_fixup(b, st);
             LAUNCH_CHECK();
             return HRW_OK;

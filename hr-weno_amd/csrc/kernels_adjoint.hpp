@@ -316,10 +316,11 @@ __device__ __forceinline__ double recon3_vjp(double eps12, const double *x, cons
     return abar;
 }
 
-// K = 2 from the pairs (v_{i-1}, v_i) and (v_i, v_{i+1}); acc[0..2] = cells i-1..i+1.
-__device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pair2 &p0,
-                                           double &vr, double &vl, double *phi3, double alpha,
-                                           double *acc, double &abar)
+// Reverse of one K = 2 reconstruction from the pairs (v_{i-1}, v_i) and (v_i, v_{i+1});
+// acc[0..2] = cells i-1..i+1.  The cotangents of the faces come from face_cot as for weno5_vjp.
+template <class FaceCot>
+__device__ __forceinline__ void weno3_vjp(double eps, const Pair2 &p1, const Pair2 &p0,
+                                          FaceCot &&face_cot, double *acc)
 {
     const double q0 = eps + p0.B, q1 = eps + p1.B;
     const double Q0 = q0 * q0, Q1 = q1 * q1;
@@ -328,12 +329,10 @@ __device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pa
     const double nr = fma_(a0, p0.S, Q0 * p1.Rr);
     const double nl = fma_(Q1, p0.Lr, b1 * p1.S);
     const double ir = rcp_nr(sr), il = rcp_nr(sl);
-    vr = nr * ir;
-    vl = nl * il;
-    const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
-    const double gr = Gp * (vr + alpha), gl = Gm * (vl - alpha);
-    abar = fma_(Gp, vr, -(Gm * vl));
-    const double sgr = gr * ir, sgl = gl * il;
+    const double vr = nr * ir;
+    const double vl = nl * il;
+    double sgr, sgl;
+    face_cot(vr, vl, ir, il, sgr, sgl);
     const double vr2 = 2.0 * vr, vl2 = 2.0 * vl;
     const double iq = rcp_nr(q0 * q1);
     // q-bar_r = -2/q_r [ (2x right weight numerator) sgr (cand - 2vr) + ... ]
@@ -354,6 +353,21 @@ __device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pa
         acc[0] += Sb - Rb - dd;
         acc[1] += fma_(3.0, Rb, Sb + dd);
     }
+}
+
+// The Lax-Friedrichs form (halved phi3 as for recon3_vjp); abar = the cell's share of alpha-bar.
+__device__ __forceinline__ void recon2_vjp(double eps, const Pair2 &p1, const Pair2 &p0,
+                                           double *phi3, double alpha, double *acc, double &abar)
+{
+    weno3_vjp(eps, p1, p0,
+              [&](double vr, double vl, double ir, double il, double &sgr, double &sgl) {
+                  const double Gp = phi3[2] - phi3[1], Gm = phi3[1] - phi3[0];
+                  const double gr = Gp * (vr + alpha), gl = Gm * (vl - alpha);
+                  abar = fma_(Gp, vr, -(Gm * vl));
+                  sgr = gr * ir;
+                  sgl = gl * il;
+              },
+              acc);
 }
 
 // One wavefront's chunk of a line (see the header comment).  Returns the lane's share of
@@ -508,8 +522,8 @@ __device__ __forceinline__ double chunk_adjoint(const AdjArgs &a, const double *
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             const Pair2 p0 = pair2(v[t + 1], v[t + 2]);
-            double vr, vl, ab;
-            recon2_vjp(a.eps, p1, p0, vr, vl, phi + t, alpha, acc + t, ab);
+            double ab;
+            recon2_vjp(a.eps, p1, p0, phi + t, alpha, acc + t, ab);
             if (owned && c0 + t < n) {
                 abar += ab;
                 if (ALPHAMAX && fabs(v[t + 1]) == alpha && cnd == 0xffffffffu) cnd = (unsigned)(c0 + t);

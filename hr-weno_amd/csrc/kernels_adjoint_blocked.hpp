@@ -1,6 +1,6 @@
 // kernels_adjoint_blocked.hpp -- register-blocked form of the general reverse pass for the
-// fifth-order reconstruction (K = 3): the endpoint Godunov flux, or Lax-Friedrichs with a fixed
-// alpha or alpha = max|s|; periodic or zero-gradient ends
+// third- and fifth-order reconstructions (K = 2, 3): the endpoint Godunov flux, or
+// Lax-Friedrichs with a fixed alpha or alpha = max|s|; periodic or zero-gradient ends
 // (example/example1_burgers_1d_fv.f90:72-109; src/hrweno_weno.f90:174-216 per cell;
 // src/hrweno_fluxes.f90:42-43, 66-74).  It computes what rhs_vjp_kernel computes,
 //     out = ey1 y1 + ey2 y2 + ex0 x0 + ex1 x1 + ch J(s)^T (b1 y1 + b2 y2),
@@ -10,12 +10,12 @@
 // from c0 = W + (l-2)R and their window halo from its neighbours (DPP wave shifts).  Lanes 0,
 // 1, 62 and 63 are halo lanes: 1 and 62 run the reverse of the cells whose windows spill into
 // the wavefront's first and last cells, 0 and 63 only supply the faces and phi those need (and
-// load the H cells beyond the wavefront), so no lane reconstructs a cell twice.  Each lane
+// load the H = K-1 cells beyond the wavefront), so no lane reconstructs a cell twice.  Each lane
 // forms the faces of its cells, the interface cotangents
 // (phi = (b1 y1 + b2 y2)/w, the cotangent of F_{p+1/2} is phi_{p+1} - phi_p):
 //     vr-bar_p = (phi_{p+1} - phi_p) dF/dvm (vr_p, vl_{p+1}),
 //     vl-bar_p = (phi_p - phi_{p-1}) dF/dvp (vr_{p-1}, vl_p),
-// runs the reverse of each reconstruction into R + 4 accumulators and hands the two that spill
+// runs the reverse of each reconstruction into R + 2H accumulators and hands the H that spill
 // past each end to the neighbouring lane.  No LDS tile and no barrier (the alpha = max|s| form
 // has one, for its block sum at the end).
 // Ends: with periodic ends every position stands for its periodic image (every (cell, offset)
@@ -59,11 +59,13 @@ struct BvjpArgs {
 
 // Returns the lane's share of alpha-bar over its owned cells (AMAX); *cand = the first owned cell
 // with |s| == alpha (0xffffffff: none).
-template <int FLUX, bool PERIODIC, bool INTERIOR, bool AMAX>
+template <int K, int FLUX, bool PERIODIC, bool INTERIOR, bool AMAX>
 __device__ __forceinline__ double bvjp_chunk(const BvjpArgs &a, long long W, int lane, double alpha,
                                              unsigned *cand)
 {
-    constexpr int R = kBvjpR, H = 2, NV = R + 2 * H;
+    static_assert(K == 2 || K == 3, "third- and fifth-order reconstructions");
+    constexpr int R = kBvjpR, H = K - 1, NV = R + 2 * H;
+    static_assert(H <= R, "the spills reach the neighbouring lanes only");
     const long long n = a.n;
     const long long c0 = W + (long long)(lane - 2) * R;
     // the edge lanes' window halo beyond the wavefront: the H cells left of lane 0's cells
@@ -137,16 +139,22 @@ __device__ __forceinline__ double bvjp_chunk(const BvjpArgs &a, long long W, int
         for (int i = 0; i < H; ++i) v[H + R + i] = hv[i];
     }
 
-    // faces of the lane's cells (triples shared between neighbouring cells) and the
+    // faces of the lane's cells (triples or pairs shared between neighbouring cells) and the
     // neighbours' (lanes 0 and 63 get their own back: unused)
-    const double eps12 = 12.0 * a.eps;
+    [[maybe_unused]] const double eps12 = 12.0 * a.eps;
     double vr[R], vl[R];
-    {
+    if constexpr (K == 3) {
         Tri3 T[R + 2];
 #pragma unroll
         for (int j = 0; j < R + 2; ++j) T[j] = tri3(v[j], v[j + 1], v[j + 2]);
 #pragma unroll
         for (int t = 0; t < R; ++t) recon3<false, true>(eps12, T[t], T[t + 1], T[t + 2], vl[t], vr[t]);
+    } else {
+        Pair2 P[R + 1];
+#pragma unroll
+        for (int j = 0; j < R + 1; ++j) P[j] = pair2(v[j], v[j + 1]);
+#pragma unroll
+        for (int t = 0; t < R; ++t) recon2(a.eps, P[t], P[t + 1], vl[t], vr[t]);
     }
     const double vr_prev = shfl_up1(vr[R - 1]), vl_next = shfl_down1(vl[0]);
     // flux derivatives at faces f = 0..R (between local cells f-1 and f)
@@ -181,12 +189,14 @@ __device__ __forceinline__ double bvjp_chunk(const BvjpArgs &a, long long W, int
     for (int t = 0; t < R; ++t) {
         const double gr = (phi[t + 2] - phi[t + 1]) * dm[t + 1];
         const double gl = (phi[t + 1] - phi[t]) * dp[t];
-        weno5_vjp(eps12, v + t,
-                  [&](double, double, double ir, double il, double &sgr, double &sgl) {
-                      sgr = gr * ir;
-                      sgl = gl * il;
-                  },
-                  acc + t);
+        auto face_cot = [&](double, double, double ir, double il, double &sgr, double &sgl) {
+            sgr = gr * ir;
+            sgl = gl * il;
+        };
+        if constexpr (K == 3)
+            weno5_vjp(eps12, v + t, face_cot, acc + t);
+        else
+            weno3_vjp(a.eps, pair2(v[t], v[t + 1]), pair2(v[t + 1], v[t + 2]), face_cot, acc + t);
     }
     // the spills into the neighbours' cells
 #pragma unroll
@@ -197,11 +207,16 @@ __device__ __forceinline__ double bvjp_chunk(const BvjpArgs &a, long long W, int
         acc[H + i] += fromL;
     }
     if constexpr (!INTERIOR && !PERIODIC) {
-        // window positions past an end are the end cell: positions c0-2, c0-1 of the lane
-        // holding cell 0 (only its own cells' windows reach them) and positions n, n+1 of the
+        // window positions past an end are the end cell: positions c0-H .. c0-1 of the lane
+        // holding cell 0 (only its own cells' windows reach them) and positions n .. of the
         // lane holding cell n-1 (its own cells' and, through the exchange, its left neighbour's;
         // the cells right of n-1 are images with phi = 0 on both sides, which add nothing)
-        if (c0 == 0) acc[H] += acc[0] + acc[1];
+        if (c0 == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int m = 0; m < H; ++m) t += acc[m];
+            acc[H] += t;
+        }
         const long long jl = n - 1 - c0;
         if (jl >= 0 && jl < R) {
             double t = 0.0;
@@ -232,11 +247,11 @@ __device__ __forceinline__ double bvjp_chunk(const BvjpArgs &a, long long W, int
 // One wavefront per chunk of 60R output cells, 4 waves per SIMD (up to 128 VGPRs: 118-128
 // without spills; left at 3, the alpha = max|s| form took 130).  Asking for 5 (96 VGPRs) spills
 // 33-48 registers.
-template <int FLUX, bool PERIODIC, bool AMAX>
+template <int K, int FLUX, bool PERIODIC, bool AMAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void rhs_vjp5_kernel(
     BvjpArgs a)
 {
-    constexpr int R = kBvjpR, H = 2;
+    constexpr int R = kBvjpR, H = K - 1;
     __shared__ double s_ab[4];
     __shared__ unsigned s_cnd[4];
     const long long c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -257,9 +272,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
         const long long W = c * 60LL * R;
         // every cell the wavefront touches lies in [W - 2R - H, W + 62R + H - 1]
         if (W - 2 * R - H >= 0 && W + 62LL * R + H - 1 <= a.n - 1)
-            ab = bvjp_chunk<FLUX, PERIODIC, true, AMAX>(a, W, lane, alpha, &cnd);
+            ab = bvjp_chunk<K, FLUX, PERIODIC, true, AMAX>(a, W, lane, alpha, &cnd);
         else
-            ab = bvjp_chunk<FLUX, PERIODIC, false, AMAX>(a, W, lane, alpha, &cnd);
+            ab = bvjp_chunk<K, FLUX, PERIODIC, false, AMAX>(a, W, lane, alpha, &cnd);
     }
     if constexpr (AMAX) {
         // the block's share of alpha-bar in a fixed order, and its first cell at the maximum

@@ -51,9 +51,10 @@ void launch_adjoint_stage(int R, bool alphamax, const AdjArgs &g, hipStream_t s)
 void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjArgs &g,
                     hipStream_t s);
 void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s);
-// The same for K = 3, register-blocked: one wavefront per 60*kBvjpR cells (and its alpha-bar
-// correction for alpha = max|s|).
-void launch_rhs_vjp5(int flux, bool periodic, bool alphamax, const BvjpArgs &g, hipStream_t s);
+// The same for K = 2 or 3, register-blocked: one wavefront per 60*kBvjpR cells (and its
+// alpha-bar correction for alpha = max|s|).
+void launch_rhs_vjp5(int k, int flux, bool periodic, bool alphamax, const BvjpArgs &g,
+                     hipStream_t s);
 void launch_rhs_vjp5_fixup(const BvjpArgs &g, hipStream_t s);
 
 }  // namespace hrw

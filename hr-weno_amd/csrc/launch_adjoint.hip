@@ -44,22 +44,30 @@ void launch_rhs_vjp(int k, int flux, bool periodic, bool alphamax, const GenAdjA
     else lgen_k<3>(flux, periodic, alphamax, g, s);
 }
 
-void launch_rhs_vjp5(int flux, bool periodic, bool alphamax, const BvjpArgs &g, hipStream_t s)
+template <int K>
+static void lvjp5(int flux, bool periodic, bool alphamax, const BvjpArgs &g, hipStream_t s)
 {
-    if (g.nchunks <= 0) return;
     const unsigned grid = (unsigned)((g.nchunks + 3) / 4);
     if (flux == LF) {
         if (alphamax) {
-            if (periodic) rhs_vjp5_kernel<LF, true, true><<<grid, 256, 0, s>>>(g);
-            else rhs_vjp5_kernel<LF, false, true><<<grid, 256, 0, s>>>(g);
+            if (periodic) rhs_vjp5_kernel<K, LF, true, true><<<grid, 256, 0, s>>>(g);
+            else rhs_vjp5_kernel<K, LF, false, true><<<grid, 256, 0, s>>>(g);
         } else {
-            if (periodic) rhs_vjp5_kernel<LF, true, false><<<grid, 256, 0, s>>>(g);
-            else rhs_vjp5_kernel<LF, false, false><<<grid, 256, 0, s>>>(g);
+            if (periodic) rhs_vjp5_kernel<K, LF, true, false><<<grid, 256, 0, s>>>(g);
+            else rhs_vjp5_kernel<K, LF, false, false><<<grid, 256, 0, s>>>(g);
         }
     } else {
-        if (periodic) rhs_vjp5_kernel<GODUNOV, true, false><<<grid, 256, 0, s>>>(g);
-        else rhs_vjp5_kernel<GODUNOV, false, false><<<grid, 256, 0, s>>>(g);
+        if (periodic) rhs_vjp5_kernel<K, GODUNOV, true, false><<<grid, 256, 0, s>>>(g);
+        else rhs_vjp5_kernel<K, GODUNOV, false, false><<<grid, 256, 0, s>>>(g);
     }
+}
+
+void launch_rhs_vjp5(int k, int flux, bool periodic, bool alphamax, const BvjpArgs &g,
+                     hipStream_t s)
+{
+    if (g.nchunks <= 0) return;
+    if (k == 2) lvjp5<2>(flux, periodic, alphamax, g, s);
+    else lvjp5<3>(flux, periodic, alphamax, g, s);
 }
 
 void launch_rhs_vjp5_fixup(const BvjpArgs &g, hipStream_t s)

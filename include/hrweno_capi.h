@@ -186,8 +186,8 @@ int hrw_problem_pbe2d(int n1, int n2, int k, double eps, int bc, double x1min, d
  * variant 0 = register-blocked kernel with cells_per_lane (4, 6 or 8) consecutive cells per
  * lane, one chunk per wavefront; with cells_per_lane 0 (the default) the form and the cells per
  * lane are chosen by the line length (2^21 cells or more: variant 2 with 4; 2^20 or more: 6),
- * variant 1 = LDS-tile kernel (for the reverse passes of the adjoint too: the K = 3 register-
- * blocked reverse pass is then not used), variant 2 = the register-blocked kernel run persistently with
+ * variant 1 = LDS-tile kernel (for the reverse passes of the adjoint too: the K = 2, 3
+ * register-blocked reverse pass is then not used), variant 2 = the register-blocked kernel run persistently with
  * the next chunk's loads issued ahead of the current chunk's arithmetic (lines with a width
  * array take the variant 0 kernel).  Variants 0 and 2 give identical results; variant 1
  * agrees within the FAST tolerance.  For a 2D problem

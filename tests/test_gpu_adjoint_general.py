@@ -140,8 +140,9 @@ def test_conserved_total_and_fixed_ends_on_device(hh, dev, k, flux):
 
 @pytest.mark.parametrize("flux,bc,alpha", [(GOD, PER, -1.0), (GOD, ZG, -1.0), (LF, PER, 1.7),
                                            (LF, ZG, 1.7), (LF, PER, -1.0), (LF, ZG, -1.0)])
-def test_blocked_reverse_matches_tile_reverse(hh, dev, flux, bc, alpha):
-    """K = 3 takes the register-blocked reverse pass (120 cells per wavefront; with alpha =
+@pytest.mark.parametrize("k", [2, 3])
+def test_blocked_reverse_matches_tile_reverse(hh, dev, flux, bc, alpha, k):
+    """K = 2 and 3 take the register-blocked reverse pass (120 cells per wavefront; with alpha =
     max|s| its alpha-bar shares per block and the fixup launch); variant 1 of the problem keeps
     the LDS-tile pass.  With the EXACT forward (the
     same states under both) the gradients agree to rounding: rktvd (one cotangent and pointwise
@@ -158,7 +159,7 @@ def test_blocked_reverse_matches_tile_reverse(hh, dev, flux, bc, alpha):
         got = {}
         for variant in (0, 1):
             def prob():
-                return hh.Burgers1D(n, k=3, flux=flux, bc=bc, alpha=alpha,
+                return hh.Burgers1D(n, k=k, flux=flux, bc=bc, alpha=alpha,
                                     arith="exact").tune(variant)
             r = []
             if not (flux == LF and bc == PER):
