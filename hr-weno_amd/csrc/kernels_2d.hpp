@@ -45,7 +45,7 @@ __device__ __forceinline__ void recon(const double *x, double eps, double &vl, d
     if constexpr (!FAST) {
         weno_cell_exact<K>(x, eps, nullptr, vl, vr);
     } else if constexpr (K == 3) {   // the 1D kernel's form (scaled triples, reciprocals)
-        recon3(12.0 * eps, tri3(x[0], x[1], x[2]), tri3(x[1], x[2], x[3]), tri3(x[2], x[3], x[4]), vl, vr);
+        recon3<false, true>(12.0 * eps, tri3(x[0], x[1], x[2]), tri3(x[1], x[2], x[3]), tri3(x[2], x[3], x[4]), vl, vr);
     } else if constexpr (K == 2) {
         recon2(eps, pair2(x[0], x[1]), pair2(x[1], x[2]), vl, vr);
     } else {
@@ -76,7 +76,7 @@ struct ColumnRecon {
     {
         if constexpr (FAST && K == 3) {
             const Tri3 t0 = tri3(cw[2], cw[3], cw[4]);
-            recon3(12.0 * eps, t2, t1, t0, vl, vr);
+            recon3<false, true>(12.0 * eps, t2, t1, t0, vl, vr);
             t2 = t1;
             t1 = t0;
         } else if constexpr (FAST && K == 2) {
@@ -257,7 +257,7 @@ __device__ __forceinline__ void pbe2d_wave(const Stage2DArgs &a)
             tr.bA = from_right(tm.bA, tm.bA);
             tr.P0 = from_right(tm.P0, tm.P0);
             tr.Pm = from_right(tm.Pm, tm.Pm);
-            recon3(12.0 * a.eps, tl, tm, tr, vl1, vr1);
+            recon3<false, true>(12.0 * a.eps, tl, tm, tr, vl1, vr1);
         } else {
             double xw[2 * K - 1];
             x1_window<K>(uc, o.halo, xw);
