@@ -976,7 +976,7 @@ extern "C" int hrw_problem_rhs(hrw_problem_t p, double t, const double *u, doubl
 // ------------------------------------------------------------------ integrators -------
 // ---- the general reverse pass of a 1D Burgers RHS (either flux, either kind of ends) ------
 // Scratch of rhs_vjp_kernel: a slot set for max|s|, one alpha-bar partial per block and the
-// argmax word (INT_MAX between passes: every fixup resets it).
+// argmax word (zero and INT_MAX between passes: every fixup clears and resets them).
 struct GenScratch {
     unsigned long long *aset = nullptr;
     double *partial = nullptr;
@@ -998,6 +998,9 @@ struct GenScratch {
         CALL(dalloc(&argmax, 1));
         const int none = INT_MAX;
         HIP_TRY(hipMemcpy(argmax, &none, sizeof none, hipMemcpyHostToDevice));
+        const std::vector<unsigned long long> zero((size_t)kNSlot * kSlotStride, 0ull);
+        HIP_TRY(hipMemcpy(aset, zero.data(), zero.size() * sizeof(unsigned long long),
+                          hipMemcpyHostToDevice));
         return HRW_OK;
     }
 };
@@ -1050,7 +1053,6 @@ static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const 
             b.n = p->neq;
             b.nchunks = (p->neq + 60LL * kBvjpR - 1) / (60LL * kBvjpR);
             if (amax) {
-                clear_slots_kernel<<<1, 256, 0, st>>>(g.aset, 1);
                 maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, st>>>(s, p->neq, g.aset);
                 b.aset = g.aset;
                 b.partial = g.partial;
@@ -1083,7 +1085,6 @@ static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const 
     a.eps = p->eps;
     a.alpha = p->alpha;
     if (amax) {
-        clear_slots_kernel<<<1, 256, 0, st>>>(g.aset, 1);
         maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, st>>>(s, p->neq, g.aset);
         a.aset = g.aset;
         a.partial = g.partial;

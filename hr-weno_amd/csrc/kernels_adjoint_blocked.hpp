@@ -52,7 +52,7 @@ struct BvjpArgs {
     double xmin, rx;             // linear grid
     int constw;                  // 1: every edge xmin + rx*i is exact, so every width is rx
     long long n, nchunks;
-    const unsigned long long *aset;  // alpha = max|s|: a slot set holding max|s|
+    unsigned long long *aset;        // alpha = max|s|: a slot set holding max|s| (the fixup clears it)
     double *partial;                 // alpha = max|s|: one alpha-bar share per block
     int *argmax;                     // alpha = max|s|: first cell with |s| == alpha (INT_MAX: none)
 };

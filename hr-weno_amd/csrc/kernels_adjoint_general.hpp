@@ -52,7 +52,7 @@ struct GenAdjArgs {
     double ch;                       // the scale of J^T (the stage's c*h)
     long long n;
     double xmin, rx, eps, alpha;     // linear grid; alpha: Lax-Friedrichs with a fixed alpha
-    const unsigned long long *aset;  // alpha = max|s|: a slot set holding max|s|
+    unsigned long long *aset;        // alpha = max|s|: a slot set holding max|s| (the fixup clears it)
     double *partial;                 // alpha = max|s|: one alpha-bar partial sum per block
     int *argmax;                     // alpha = max|s|: first cell with |s| == alpha (INT_MAX: none)
 };
@@ -338,9 +338,11 @@ __global__ __launch_bounds__(kGenTile) void rhs_vjp_kernel(GenAdjArgs a)
 }
 
 // The alpha-bar correction of a finished rhs_vjp_kernel pass: out[j] += ch * (sum of the block
-// partials) * sign(s[j]) at the first cell j attaining max|s|; the argmax slot is reset.
+// partials) * sign(s[j]) at the first cell j attaining max|s|; the argmax slot is reset and the
+// max|s| slot set cleared for the next pass (which reads them only after this launch).
 static __global__ __launch_bounds__(kGenTile) void rhs_vjp_alpha_fixup(const double *partial,
                                                                      int nblocks, int *argmax,
+                                                                     unsigned long long *aset,
                                                                      const double *s, double *out,
                                                                      double ch)
 {
@@ -362,6 +364,7 @@ static __global__ __launch_bounds__(kGenTile) void rhs_vjp_alpha_fixup(const dou
         }
         *argmax = INT_MAX;
     }
+    if (t < kNSlot) aset[t * kSlotStride] = 0ull;
 }
 
 }  // namespace hrw

@@ -73,13 +73,13 @@ void launch_rhs_vjp5(int k, int flux, bool periodic, bool alphamax, const BvjpAr
 void launch_rhs_vjp5_fixup(const BvjpArgs &g, hipStream_t s)
 {
     const int nblocks = (int)((g.nchunks + 3) / 4);
-    rhs_vjp_alpha_fixup<<<1, kGenTile, 0, s>>>(g.partial, nblocks, g.argmax, g.s, g.out, g.ch);
+    rhs_vjp_alpha_fixup<<<1, kGenTile, 0, s>>>(g.partial, nblocks, g.argmax, g.aset, g.s, g.out, g.ch);
 }
 
 void launch_rhs_vjp_fixup(const GenAdjArgs &g, hipStream_t s)
 {
     const int nblocks = (int)((g.n + kGenTile - 1) / kGenTile);
-    rhs_vjp_alpha_fixup<<<1, kGenTile, 0, s>>>(g.partial, nblocks, g.argmax, g.s, g.out, g.ch);
+    rhs_vjp_alpha_fixup<<<1, kGenTile, 0, s>>>(g.partial, nblocks, g.argmax, g.aset, g.s, g.out, g.ch);
 }
 
 void launch_adjoint_stage(int R, bool alphamax, const AdjArgs &g, hipStream_t s)
