@@ -1014,7 +1014,8 @@ struct GenTerm {
 // (variant 1); K = 1 the LDS-tile pass (rhs_vjp_kernel).
 static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const double *s,
                    const double *y1, double b1, const double *y2, double b2,
-                   std::initializer_list<GenTerm> terms, double *out, double ch)
+                   std::initializer_list<GenTerm> terms, double *out, double ch,
+                   const double *alpha_dev = nullptr)
 {
     const bool amax = p->alphamax();
     if ((p->k == 2 || p->k == 3) && p->variant != 1 && p->neq > 0) {
@@ -1053,7 +1054,8 @@ static int gen_vjp(const hrw_problem_s *p, GenScratch &g, hipStream_t st, const 
             b.n = p->neq;
             b.nchunks = (p->neq + 60LL * kBvjpR - 1) / (60LL * kBvjpR);
             if (amax) {
-                maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, st>>>(s, p->neq, g.aset);
+                if (alpha_dev) b.alpha_ptr = alpha_dev;   // max|s| recorded by the forward
+                else maxabs_kernel<<<grid_for(p->neq, kBlock * 8, 2048), kBlock, 0, st>>>(s, p->neq, g.aset);
                 b.aset = g.aset;
                 b.partial = g.partial;
                 b.argmax = g.argmax;
@@ -1267,7 +1269,8 @@ static int ode_rhs(hrw_ode_s *o, int &seq, double t, const double *us, double *L
 // stage(us, u0, L) -> uo; fused (L computed in the same pass) or callback + combine.
 static int ode_stage(hrw_ode_s *o, int &seq, int stage, double t, double h, const double *us,
                      const double *u0, double *uo, const double *uo4 = nullptr,
-                     const double *Lo4 = nullptr, double *Lout = nullptr)
+                     const double *Lo4 = nullptr, double *Lout = nullptr,
+                     double *alpha_record = nullptr)
 {
     if (o->fu) {
         double *L = Lout ? Lout : o->udot;
@@ -1287,6 +1290,7 @@ static int ode_stage(hrw_ode_s *o, int &seq, int stage, double t, double h, cons
     io.uo4 = uo4;
     io.Lo4 = Lo4;
     io.Lout = Lout;
+    io.alpha_record = alpha_record;
     CALL(run_stage(o->p, o->slots, seq, stage, h, io, o->stream));
     ++seq;
     return HRW_OK;
@@ -1342,8 +1346,10 @@ static int rk_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
 // Recording of mstvd%integrate for the adjoint: storage for the states a call adds (one
 // allocation per call, its step count known up front), a copy of a new state on the tape, or a
 // new slot for a stage to write into.  Slots are 16-byte aligned (a stride of an even number of
-// doubles), as the stage kernels' vector stores need.
-static size_t ms_stride(const hrw_ode_s *o) { return ((size_t)o->neq + 1) & ~(size_t)1; }
+// doubles), as the stage kernels' vector stores need; the double after a slot's state receives
+// the Lax-Friedrichs alpha (max|u|) of the multistep stage that reads it, which its reverse
+// reads back instead of taking the maximum again.
+static size_t ms_stride(const hrw_ode_s *o) { return ((size_t)o->neq + 2) & ~(size_t)1; }
 static int ms_reserve(hrw_ode_s *o, int64_t nstates)
 {
     if (nstates <= 0) return HRW_OK;
@@ -1464,9 +1470,9 @@ static int ms_integrate(hrw_ode_s *o, double *u, double *t, double tout, double 
             const int h4 = o->head;
             double *next = nullptr;
             int rc = ms_take(o, &next);
-            if (rc == HRW_OK)
+            if (rc == HRW_OK)   // (S_m is tape entry m, m = the steps recorded so far)
                 rc = ode_stage(o, seq, MS3, *t, dt, o->S, o->S, next, o->ring_u[h4], o->ring_L[h4],
-                               o->ring_L[h4]);
+                               o->ring_L[h4], o->ms_tape[o->ms_dts.size()] + o->neq);
             if (rc != HRW_OK) {
                 (void)settle();
                 return rc;
@@ -2839,7 +2845,7 @@ static int ms_adjoint(hrw_ode_s *o, double *lambda)
         const double *Sm = o->ms_tape[(size_t)m];
         if (m >= 4) {
             CALL(gen_vjp(p, o->gen, st, Sm, lp, 50.0 * dm / 32.0, l5, 10.0 * d5 / 32.0,
-                         {{lp, 25.0 / 32.0}, {l5, 7.0 / 32.0}}, L(m), 1.0));
+                         {{lp, 25.0 / 32.0}, {l5, 7.0 / 32.0}}, L(m), 1.0, Sm + o->neq));
         } else {
             const double *Y1 = o->ms_start[2 * m], *Y2 = o->ms_start[2 * m + 1];
             // u' = (u + 2 Y2 + 2 dt L(Y2))/3, Y2 = (3u + Y1 + dt L(Y1))/4, Y1 = u + dt L(u)

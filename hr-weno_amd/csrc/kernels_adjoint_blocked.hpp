@@ -53,6 +53,7 @@ struct BvjpArgs {
     int constw;                  // 1: every edge xmin + rx*i is exact, so every width is rx
     long long n, nchunks;
     unsigned long long *aset;        // alpha = max|s|: a slot set holding max|s| (the fixup clears it)
+    const double *alpha_ptr;         // alpha = max|s|: if set, max|s| itself (aset is not read)
     double *partial;                 // alpha = max|s|: one alpha-bar share per block
     int *argmax;                     // alpha = max|s|: first cell with |s| == alpha (INT_MAX: none)
 };
@@ -258,13 +259,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
     const int lane = threadIdx.x & 63;
     double alpha = a.alpha;
     if constexpr (AMAX) {
-        unsigned long long m = 0ull;
+        if (a.alpha_ptr) {
+            alpha = *a.alpha_ptr;
+        } else {
+            unsigned long long m = 0ull;
 #pragma unroll
-        for (int i = 0; i < kNSlot; ++i) {
-            const unsigned long long x = a.aset[i * kSlotStride];
-            m = x > m ? x : m;
+            for (int i = 0; i < kNSlot; ++i) {
+                const unsigned long long x = a.aset[i * kSlotStride];
+                m = x > m ? x : m;
+            }
+            alpha = __longlong_as_double((long long)m);
         }
-        alpha = __longlong_as_double((long long)m);
     }
     double ab = 0.0;
     unsigned cnd = 0xffffffffu;
