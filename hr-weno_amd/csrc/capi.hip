@@ -2628,9 +2628,10 @@ static int adjoint_stage(hrw_adjoint_s *a, int64_t s, int64_t q, int k, const do
                          double *out, double ca, double cc, const double *x1, double e1,
                          const double *x2, double e2)
 {
-    if (a->general)   // (its alpha-bar correction is applied within the pass: nothing owed)
+    if (a->general)   // (its alpha-bar correction is applied within the pass: nothing owed;
+                      // the alpha of X[q], max|X[q]|, was recorded by the stage that read it)
         return gen_vjp(a->p, a->gen, a->strm(), seg_state(a, s, q, k), lam, 1.0, nullptr, 0.0,
-                       {{lam, ca}, {x1, e1}, {x2, e2}}, out, cc * a->dt);
+                       {{lam, ca}, {x1, e1}, {x2, e2}}, out, cc * a->dt, a->arec(k) + q);
     AdjArgs g{};
     g.s = seg_state(a, s, q, k);
     g.lam = lam;
